@@ -1,0 +1,265 @@
+"""hptt_amd -- MI355X (gfx950) tensor transposition with HPTT's plan interface.
+
+    B[perm(i)] = alpha * A[i] + beta * B[perm(i)]
+
+Python mirror of the reference's C++ interface (inc/hptt/hptt.h:85-93,
+inc/hptt/impl/hptt_trans.h:12-36) over the C ABI in include/hptt_amd.h::
+
+    plan = hptt_amd.create_plan(A, B, in_size, perm, alpha, beta, num_threads)
+    plan.exec()                 # enqueued on the plan's HIP stream
+    plan.reset_data(A2, B2)     # same sizes / perm / coefficients
+
+A and B are device buffers: torch tensors on a ROCm device, or raw device
+addresses (int) together with ``dtype=``.  Layout is column-major (dim 0 is
+stride-1), as in the reference.  ``create_plan`` returns None on invalid
+arguments, like the reference's nullptr (inc/hptt/hptt.h:71-83);
+``last_error()`` says why.
+
+The library (hptt_amd/libhptt_amd.so) is required: there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhptt_amd.so")
+
+OK, EINVAL, ENOMEM, EHIP, EUNSUPPORTED = 0, 1, 2, 3, 4
+F32, F64, C64, C128 = 0, 1, 2, 3
+ELEM_BYTES = {F32: 4, F64: 8, C64: 8, C128: 16}
+PATHS = {0: "tile", 1: "tile-shared", 2: "wide-copy", 3: "axpby", 4: "gather"}
+
+_lib = None
+
+
+class HpttError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__("hptt_amd status %d: %s" % (status, msg))
+        self.status = status
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("elements", ctypes.c_uint64), ("elem_bytes", ctypes.c_uint32),
+        ("update", ctypes.c_int32), ("path", ctypes.c_int32),
+        ("fused_rank", ctypes.c_int32), ("algorithmic_bytes", ctypes.c_uint64),
+        ("grid", ctypes.c_uint32), ("lds_bytes", ctypes.c_uint32),
+        ("tile_r", ctypes.c_uint32), ("tile_c", ctypes.c_uint32),
+        ("tile_o", ctypes.c_uint32), ("vec_r", ctypes.c_uint32),
+        ("vec_w", ctypes.c_uint32)]
+
+
+def lib() -> ctypes.CDLL:
+    """Load libhptt_amd.so (built by ``python -m hptt_amd.build``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("hptt_amd: %s is missing -- build it with "
+                           "`python -m hptt_amd.build`" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u32, u32p = ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    f, d = ctypes.c_float, ctypes.c_double
+    for name, real in (("s", f), ("d", d), ("c", f), ("z", d)):
+        fn = getattr(L, "hpttx_plan_create_" + name)
+        fn.restype = ctypes.c_int
+        fn.argtypes = [vp, vp, u32, u32p, u32p, real, real, u32, d, u32p, u32p, vp, pp]
+    for name, real in (("c", f), ("z", d)):
+        fn = getattr(L, "hpttx_plan_create_%s_cplx" % name)
+        fn.restype = ctypes.c_int
+        fn.argtypes = [vp, vp, u32, u32p, u32p, real, real, real, real, u32, d, u32p,
+                       u32p, vp, pp]
+    for name in ("hpttx_plan_exec", "hpttx_plan_sync"):
+        getattr(L, name).restype = ctypes.c_int
+        getattr(L, name).argtypes = [vp]
+    L.hpttx_plan_exec_on.restype = ctypes.c_int
+    L.hpttx_plan_exec_on.argtypes = [vp, vp]
+    L.hpttx_plan_reset_data.restype = ctypes.c_int
+    L.hpttx_plan_reset_data.argtypes = [vp, vp, vp]
+    L.hpttx_plan_set_stream.restype = ctypes.c_int
+    L.hpttx_plan_set_stream.argtypes = [vp, vp]
+    L.hpttx_plan_describe.restype = ctypes.c_int
+    L.hpttx_plan_describe.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+    L.hpttx_plan_get_info.restype = ctypes.c_int
+    L.hpttx_plan_get_info.argtypes = [vp, ctypes.POINTER(PlanInfo)]
+    L.hpttx_plan_destroy.restype = None
+    L.hpttx_plan_destroy.argtypes = [vp]
+    L.hpttx_last_error.restype = ctypes.c_char_p
+    L.hpttx_last_error.argtypes = []
+    L.hpttx_version.restype = ctypes.c_int
+    L.hpttx_version.argtypes = []
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return lib().hpttx_last_error().decode()
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        raise HpttError(rc, last_error())
+
+
+def _u32(seq):
+    if seq is None or len(seq) == 0:
+        return None
+    return (ctypes.c_uint32 * len(seq))(*[int(v) for v in seq])
+
+
+def _torch_dtype_code(t):
+    import torch
+    return {torch.float32: F32, torch.float64: F64, torch.complex64: C64,
+            torch.complex128: C128}.get(t.dtype)
+
+
+def _addr_and_dtype(x, dtype):
+    """Device address and dtype code of a torch tensor or a raw address."""
+    if isinstance(x, int):
+        if dtype is None:
+            raise TypeError("dtype= is required when passing raw device addresses")
+        return x, dtype
+    code = _torch_dtype_code(x)
+    if code is None:
+        raise TypeError("unsupported tensor dtype %s" % (x.dtype,))
+    if not x.is_cuda:
+        raise TypeError("hptt_amd plans take device tensors")
+    return int(x.data_ptr()), code
+
+
+def dtype_code(dtype) -> int:
+    """Map 's'/'d'/'c'/'z', numpy or torch dtypes to a dtype code."""
+    if isinstance(dtype, int):
+        return dtype
+    if isinstance(dtype, str) and dtype in "sdcz" and len(dtype) == 1:
+        return "sdcz".index(dtype)
+    name = str(dtype).replace("torch.", "")
+    return {"float32": F32, "float64": F64, "complex64": C64, "complex128": C128}[name]
+
+
+class Plan:
+    """A transposition plan (reference CGraphTransPackBase<T>,
+    inc/hptt/impl/hptt_trans.h:12-36).  exec() enqueues and returns."""
+
+    def __init__(self, handle: int, dtype: int, keep=()):
+        self._h = ctypes.c_void_p(handle)
+        self.dtype = dtype
+        self._keep = list(keep)
+
+    @property
+    def handle(self) -> int:
+        return self._h.value
+
+    def exec(self) -> None:
+        _check(lib().hpttx_plan_exec(self._h))
+
+    __call__ = exec
+
+    def exec_on(self, stream) -> None:
+        _check(lib().hpttx_plan_exec_on(self._h, _stream_ptr(stream)))
+
+    def sync(self) -> None:
+        _check(lib().hpttx_plan_sync(self._h))
+
+    def reset_data(self, in_data, out_data, dtype=None) -> None:
+        a, _ = _addr_and_dtype(in_data, dtype if dtype is not None else self.dtype)
+        b, _ = _addr_and_dtype(out_data, dtype if dtype is not None else self.dtype)
+        self._keep = [in_data, out_data]
+        _check(lib().hpttx_plan_reset_data(self._h, a, b))
+
+    def set_stream(self, stream) -> None:
+        _check(lib().hpttx_plan_set_stream(self._h, _stream_ptr(stream)))
+
+    def describe(self) -> str:
+        n = lib().hpttx_plan_describe(self._h, None, 0)
+        buf = ctypes.create_string_buffer(max(n, 1))
+        lib().hpttx_plan_describe(self._h, buf, n)
+        return buf.value.decode()
+
+    def print_plan(self) -> None:
+        print(self.describe(), end="")
+
+    def info(self) -> PlanInfo:
+        inf = PlanInfo()
+        _check(lib().hpttx_plan_get_info(self._h, ctypes.byref(inf)))
+        return inf
+
+    # The reference pins OpenMP thread ids (cgraph_trans.tcc:71-89); a GPU plan
+    # has no host threads to pin, so these are accepted no-ops.
+    def set_thread_ids(self, thread_ids) -> None:
+        return None
+
+    def unset_thread_ids(self) -> None:
+        return None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.hpttx_plan_destroy(h)
+            self._h = ctypes.c_void_p(0)
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)  # torch.cuda.Stream
+
+
+def create_plan_checked(in_data, out_data, in_size, perm, alpha, beta,
+                        num_threads: int = 0, tuning_timeout: float = 0.0,
+                        in_outer_size=(), out_outer_size=(), stream=None,
+                        dtype=None) -> Plan:
+    """create_plan that raises HpttError (with the status code) on failure."""
+    code = dtype_code(dtype) if dtype is not None else None
+    a, code_a = _addr_and_dtype(in_data, code)
+    b, code_b = _addr_and_dtype(out_data, code)
+    if code_a != code_b:
+        raise TypeError("A and B must have the same dtype")
+    code = code_a
+    rank = len(perm)
+    out = ctypes.c_void_p()
+    L = lib()
+    args_tail = (_u32(in_outer_size), _u32(out_outer_size), _stream_ptr(stream),
+                 ctypes.byref(out))
+    cplx_coef = isinstance(alpha, complex) or isinstance(beta, complex)
+    if cplx_coef and code in (C64, C128):
+        al, be = complex(alpha), complex(beta)
+        fn = L.hpttx_plan_create_c_cplx if code == C64 else L.hpttx_plan_create_z_cplx
+        rc = fn(a, b, rank, _u32(in_size), _u32(perm), al.real, al.imag, be.real, be.imag,
+                int(num_threads), float(tuning_timeout), *args_tail)
+    else:
+        if cplx_coef:
+            raise TypeError("complex coefficients need a complex dtype")
+        fn = getattr(L, "hpttx_plan_create_" + "sdcz"[code])
+        rc = fn(a, b, rank, _u32(in_size), _u32(perm), float(alpha), float(beta),
+                int(num_threads), float(tuning_timeout), *args_tail)
+    _check(rc)
+    return Plan(out.value, code, keep=(in_data, out_data))
+
+
+def create_plan(in_data, out_data, in_size, perm, alpha, beta,
+                num_threads: int = 0, tuning_timeout: float = 0.0,
+                in_outer_size=(), out_outer_size=(), stream=None, dtype=None):
+    """Reference hptt::create_plan (inc/hptt/hptt.h:85-93): a Plan, or None
+    when the arguments are invalid (last_error() gives the reason)."""
+    try:
+        return create_plan_checked(in_data, out_data, in_size, perm, alpha, beta,
+                                   num_threads, tuning_timeout, in_outer_size,
+                                   out_outer_size, stream, dtype)
+    except HpttError:
+        return None
+
+
+def algorithmic_bytes(n_elements: int, elem_bytes: int, update: bool) -> int:
+    """Bytes one exec must move: read A, write B, read B iff beta is used."""
+    return n_elements * elem_bytes * (3 if update else 2)
+
+
+def reference_gibps(n_elements: int, elem_bytes: int, seconds: float) -> float:
+    """The reference's own throughput figure, 3*N*sizeof(T) bytes in GiB per
+    second whatever beta is (src/util/util_trans.cc:12-27)."""
+    return 3.0 * n_elements * elem_bytes / 2.0 ** 30 / seconds

@@ -1,0 +1,451 @@
+// hpttx_core.h -- problem descriptors and the per-block bodies of the
+// transposition kernels.
+//
+// Every body here is written once as a host+device function taking an explicit
+// thread id, so the HIP kernels (kernels.hip, batch.hip) run exactly this code
+// on gfx950 between __syncthreads() barriers, and the test-only emulator runs
+// the same code on the host thread by thread to check every index against the
+// buffer extents before a kernel ever touches a GPU.
+//
+// Operation (reference README.md:6-12, inc/hptt/hptt.h:18-22):
+//   B[j] = alpha*A[i] (+ beta*B[j]),  j_d = i_{perm[d]}, column-major A and B.
+// Arithmetic order (src/arch/avx2/kernel_trans_avx2.cc:67-108,
+// inc/hptt/arch/common/common_impl.h:18-33): UPDATE r = fl(fl(b*y) + fl(a*x)),
+// otherwise r = fl(a*x) and B is never read.  Contraction is off for every
+// translation unit that includes this header.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+#include <string.h>
+
+#if defined(__HIP__)
+#include <hip/hip_runtime.h>
+#define HPTTX_HD __host__ __device__ __forceinline__
+#define HPTTX_RESTRICT __restrict__
+#else
+#define HPTTX_HD inline
+#define HPTTX_RESTRICT __restrict__
+#endif
+
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+
+// Access hook: the host emulator used by the tests defines it to bounds-check
+// and count every global and LDS access; kernels compile it away.
+#ifndef HPTTX_ACCESS
+#define HPTTX_ACCESS(ptr, bytes, is_store) ((void)0)
+#endif
+
+namespace hpttx {
+
+constexpr int kMaxRank = 6;
+constexpr uint32_t kBlock = 256;   // threads per block (4 wave64)
+constexpr int kUnroll = 4;         // independent vector accesses per thread per step
+
+enum Dtype : int { kF32 = 0, kF64 = 1, kC64 = 2, kC128 = 3 };
+
+// ---------------------------------------------------------------------------
+// Division by a run-time constant: q = (mulhi(n, m) + n) >> s, exact for every
+// 32-bit n (round-up multiplier, Granlund & Montgomery 1994, 64-bit add).
+struct FastDiv {
+  uint32_t d, m, s, pad_;
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv f{};
+  f.d = d ? d : 1;
+  uint32_t s = 0;
+  while ((uint64_t(1) << s) < f.d) ++s;
+  f.s = s;
+  f.m = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << s) - f.d)) / f.d + 1);
+  return f;
+}
+
+HPTTX_HD uint32_t fdiv(uint32_t n, const FastDiv &f) {
+  uint64_t hi = ((uint64_t)n * f.m) >> 32;
+  return (uint32_t)((hi + (uint64_t)n) >> f.s);
+}
+
+// ---------------------------------------------------------------------------
+// A mixed-radix index: super-index x over up to kMaxRank dims (dim 0 fastest)
+// mapped to element offsets in A (sa) and B (sb).
+struct DimList {
+  uint32_t nd;
+  uint32_t pad_;
+  FastDiv div[kMaxRank];   // div[i].d = extent of dim i
+  int64_t sa[kMaxRank];
+  int64_t sb[kMaxRank];
+};
+
+HPTTX_HD void dl_offsets(const DimList &L, uint32_t x, int64_t &oa, int64_t &ob) {
+  int64_t a = 0, b = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxRank; ++i) {
+    if (i >= (int)L.nd) break;
+    uint32_t dig = x;
+    if (i + 1 < (int)L.nd) {
+      uint32_t q = fdiv(x, L.div[i]);
+      dig = x - q * L.div[i].d;
+      x = q;
+    }
+    a += (int64_t)dig * L.sa[i];
+    b += (int64_t)dig * L.sb[i];
+  }
+  oa = a;
+  ob = b;
+}
+
+// ---------------------------------------------------------------------------
+// TILE path.  The fused problem is split into
+//   S: a shared innermost group of VS elements, contiguous in both A and B
+//      (VS = 1 unless A's and B's stride-1 dims coincide and are short);
+//   R: a super-dim of VR indices contiguous in A right after S (A offset
+//      r*VS), scattered in B (offsets from R.sb);
+//   C: a super-dim of VC indices contiguous in B right after S (B offset
+//      c*VS), scattered in A (offsets from C.sa);
+//   O: every other dim, VO indices, offsets in both from O.
+// A block tile is TR x TC x TO of (R, C, O) indices (times VS).  It is read
+// with coalesced runs of TR*VS elements along A, staged in LDS as
+// [o][c][r*VS+s] rows of lds_row elements, and written with coalesced runs of
+// TC*VS elements along B.
+struct TileDesc {
+  uint32_t VS, VR, TR, VC, TC, VO, TO;
+  uint32_t nR, nC, nO;
+  uint32_t ntiles;
+  uint32_t lds_row;      // LDS row stride in elements (multiple of the read vector)
+  uint32_t nvr;          // vectors per read run  (TR*VS / vr)
+  uint32_t nvw;          // vectors per write run (TC*VS / vw)
+  uint32_t vr, vw;       // vector widths in elements
+  FastDiv div_nR, div_nC, div_nvr, div_TC, div_nvw, div_TR, div_VS;
+  DimList R, C, O;
+};
+
+// ROWS path.  Rows of L elements with element strides ra (A) and rb (B);
+// ra = rb = 1 is the wide-copy case (stride-1 dim kept) and the identity /
+// axpby case (a single row).  Row index -> offsets through W, rows walked in
+// B order.  A block tile is one segment of SEG elements of RB rows.
+struct RowDesc {
+  uint32_t L, SEG, nseg, nrows, RB, nrb, ntiles, v, nvs;
+  uint32_t pad_;
+  int64_t ra, rb;
+  FastDiv div_nseg, div_nvs;
+  DimList W;
+};
+
+template <typename S>
+struct Coef {
+  S ar, ai, br, bi;
+};
+
+// ---------------------------------------------------------------------------
+// Element types: NC scalar components of type S (complex = interleaved re/im).
+template <typename S, int NC>
+struct alignas(sizeof(S) * NC) Elem {
+  S c[NC];
+};
+
+typedef uint32_t u32x2 __attribute__((vector_size(8)));
+typedef uint32_t u32x4 __attribute__((vector_size(16)));
+
+template <int BYTES> struct RawVec;
+template <> struct RawVec<4> { typedef uint32_t type; };
+template <> struct RawVec<8> { typedef u32x2 type; };
+template <> struct RawVec<16> { typedef u32x4 type; };
+
+template <typename T, int V>
+HPTTX_HD void vload(const T *p, T (&out)[V]) {
+  typedef typename RawVec<sizeof(T) * V>::type R;
+  HPTTX_ACCESS(p, sizeof(R), false);
+  R r = *reinterpret_cast<const R *>(p);
+  __builtin_memcpy(&out[0], &r, sizeof(R));
+}
+
+template <typename T, int V, bool NT>
+HPTTX_HD void vstore(T *p, const T (&in)[V]) {
+  typedef typename RawVec<sizeof(T) * V>::type R;
+  R r;
+  HPTTX_ACCESS(p, sizeof(R), true);
+  __builtin_memcpy(&r, &in[0], sizeof(R));
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (NT)
+    __builtin_nontemporal_store(r, reinterpret_cast<R *>(p));
+  else
+    *reinterpret_cast<R *>(p) = r;
+#else
+  *reinterpret_cast<R *>(p) = r;
+#endif
+}
+
+// r = fl(a*x) or fl(fl(b*y) + fl(a*x)) per component; complex coefficients
+// use ax = (fl(fl(ar*xr) - fl(ai*xi)), fl(fl(ar*xi) + fl(ai*xr))), same for by.
+template <typename S, int NC, bool UPD, bool CC>
+HPTTX_HD Elem<S, NC> axpby(const Elem<S, NC> &x, const Elem<S, NC> &y,
+                           const Coef<S> &k) {
+  Elem<S, NC> r;
+  if (!CC) {
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      S ax = k.ar * x.c[i];
+      if (UPD) {
+        S by = k.br * y.c[i];
+        r.c[i] = by + ax;
+      } else {
+        r.c[i] = ax;
+      }
+    }
+  } else {
+    S p0 = k.ar * x.c[0];
+    S p1 = k.ai * x.c[NC - 1];
+    S p2 = k.ar * x.c[NC - 1];
+    S p3 = k.ai * x.c[0];
+    S axr = p0 - p1;
+    S axi = p2 + p3;
+    if (UPD) {
+      S q0 = k.br * y.c[0];
+      S q1 = k.bi * y.c[NC - 1];
+      S q2 = k.br * y.c[NC - 1];
+      S q3 = k.bi * y.c[0];
+      S byr = q0 - q1;
+      S byi = q2 + q3;
+      r.c[0] = byr + axr;
+      r.c[NC - 1] = byi + axi;
+    } else {
+      r.c[0] = axr;
+      r.c[NC - 1] = axi;
+    }
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// TILE body.
+struct TileCoord {
+  uint32_t r0, c0, o0, rlen, clen, olen;
+};
+
+HPTTX_HD uint32_t umin32(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
+HPTTX_HD TileCoord tile_coord(const TileDesc &d, uint32_t t) {
+  TileCoord tc;
+  uint32_t q = fdiv(t, d.div_nR);
+  uint32_t tr = t - q * d.nR;
+  uint32_t to = fdiv(q, d.div_nC);
+  uint32_t tcc = q - to * d.nC;
+  tc.r0 = tr * d.TR;
+  tc.c0 = tcc * d.TC;
+  tc.o0 = to * d.TO;
+  tc.rlen = umin32(d.TR, d.VR - tc.r0);
+  tc.clen = umin32(d.TC, d.VC - tc.c0);
+  tc.olen = umin32(d.TO, d.VO - tc.o0);
+  return tc;
+}
+
+// LDS footprint of a tile, in bytes: data rows then the four offset tables.
+inline size_t tile_lds_bytes(const TileDesc &d, size_t esize) {
+  size_t data = (size_t)d.TO * d.TC * d.lds_row * esize;
+  data = (data + 15) & ~(size_t)15;
+  return data + sizeof(int64_t) * ((size_t)d.TC + d.TR + 2 * (size_t)d.TO);
+}
+
+struct TileTabs {
+  int64_t *AC;  // A offset of C index c0 + c      [TC]
+  int64_t *BR;  // B offset of R index r0 + r      [TR]
+  int64_t *AO;  // A offset of O index o0 + o      [TO]
+  int64_t *BO;  // B offset of O index o0 + o      [TO]
+};
+
+HPTTX_HD TileTabs tile_tabs(const TileDesc &d, void *lds_base, size_t esize) {
+  size_t data = (size_t)d.TO * d.TC * d.lds_row * esize;
+  data = (data + 15) & ~(size_t)15;
+  int64_t *p = reinterpret_cast<int64_t *>(reinterpret_cast<char *>(lds_base) + data);
+  TileTabs t;
+  t.AC = p;
+  t.BR = p + d.TC;
+  t.AO = p + d.TC + d.TR;
+  t.BO = p + d.TC + d.TR + d.TO;
+  return t;
+}
+
+HPTTX_HD void tile_tables(const TileDesc &d, const TileCoord &tc, uint32_t tid,
+                          const TileTabs &tb) {
+  for (uint32_t i = tid; i < tc.clen; i += kBlock) {
+    int64_t a, b;
+    dl_offsets(d.C, tc.c0 + i, a, b);
+    tb.AC[i] = a;
+  }
+  for (uint32_t i = tid; i < tc.rlen; i += kBlock) {
+    int64_t a, b;
+    dl_offsets(d.R, tc.r0 + i, a, b);
+    tb.BR[i] = b;
+  }
+  for (uint32_t i = tid; i < tc.olen; i += kBlock) {
+    int64_t a, b;
+    dl_offsets(d.O, tc.o0 + i, a, b);
+    tb.AO[i] = a;
+    tb.BO[i] = b;
+  }
+}
+
+// Read pass: item k = (row, v), row = o*TC + c; one vector of VR elements
+// from A's run r0*VS + [0, rlen*VS) into LDS row `row`.
+template <typename T, int VR>
+HPTTX_HD void tile_read(const TileDesc &d, const TileCoord &tc, uint32_t tid,
+                        const T *HPTTX_RESTRICT A, T *lds, const TileTabs &tb) {
+  const uint32_t nvr = d.nvr;
+  const uint32_t items = nvr * d.TC * tc.olen;
+  const uint32_t run = tc.rlen * d.VS;
+  const int64_t abase = (int64_t)tc.r0 * d.VS;
+  for (uint32_t k0 = tid; k0 < items; k0 += kBlock * kUnroll) {
+    T buf[kUnroll][VR];
+    uint32_t lidx[kUnroll];
+    bool ok[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const uint32_t k = k0 + (uint32_t)u * kBlock;
+      const uint32_t row = fdiv(k, d.div_nvr);
+      const uint32_t v = k - row * nvr;
+      const uint32_t o = fdiv(row, d.div_TC);
+      const uint32_t c = row - o * d.TC;
+      ok[u] = (k < items) && (c < tc.clen) && (v * VR < run);
+      lidx[u] = row * d.lds_row + v * VR;
+      if (ok[u]) vload<T, VR>(A + abase + v * VR + tb.AC[c] + tb.AO[o], buf[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+      if (ok[u]) vstore<T, VR, false>(lds + lidx[u], buf[u]);
+  }
+}
+
+// Write pass: item k = (row, v), row = o*TR + r; VW elements of B's run
+// c0*VS + [0, clen*VS) gathered from LDS column r, combined with alpha/beta,
+// written as one vector.
+template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT>
+HPTTX_HD void tile_write(const TileDesc &d, const TileCoord &tc, uint32_t tid,
+                         T *HPTTX_RESTRICT B, const T *lds, const TileTabs &tb,
+                         const Coef<S> &k) {
+  const uint32_t nvw = d.nvw;
+  const uint32_t items = nvw * d.TR * tc.olen;
+  const uint32_t run = tc.clen * d.VS;
+  const int64_t bbase = (int64_t)tc.c0 * d.VS;
+  for (uint32_t k0 = tid; k0 < items; k0 += kBlock * kUnroll) {
+    T x[kUnroll][VW];
+    T y[kUnroll][VW];
+    int64_t boff[kUnroll];
+    bool ok[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const uint32_t kk = k0 + (uint32_t)u * kBlock;
+      const uint32_t row = fdiv(kk, d.div_nvw);
+      const uint32_t v = kk - row * nvw;
+      const uint32_t o = fdiv(row, d.div_TR);
+      const uint32_t r = row - o * d.TR;
+      ok[u] = (kk < items) && (r < tc.rlen) && (v * VW < run);
+      boff[u] = 0;
+      if (ok[u]) {
+        boff[u] = bbase + v * VW + tb.BR[r] + tb.BO[o];
+        const uint32_t lrow0 = o * d.TC;
+#pragma unroll
+        for (int j = 0; j < VW; ++j) {
+          const uint32_t q = v * VW + j;
+          uint32_t c = q, s = 0;
+          if (SH) {
+            c = fdiv(q, d.div_VS);
+            s = q - c * d.VS;
+          }
+          const T *src = lds + (lrow0 + c) * d.lds_row + r * d.VS + s;
+          HPTTX_ACCESS(src, sizeof(T), false);
+          x[u][j] = *src;
+        }
+        if (UPD) vload<T, VW>(B + boff[u], y[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if (!ok[u]) continue;
+      T out[VW];
+#pragma unroll
+      for (int j = 0; j < VW; ++j) {
+        const Elem<S, NC> &xe = reinterpret_cast<const Elem<S, NC> &>(x[u][j]);
+        const Elem<S, NC> &ye = reinterpret_cast<const Elem<S, NC> &>(y[u][j]);
+        Elem<S, NC> re = axpby<S, NC, UPD, CC>(xe, ye, k);
+        __builtin_memcpy(&out[j], &re, sizeof(T));
+      }
+      vstore<T, VW, NT>(B + boff[u], out);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ROWS body.
+struct RowCoord {
+  uint32_t row0, nr, seg0, slen;
+};
+
+HPTTX_HD RowCoord row_coord(const RowDesc &d, uint32_t t) {
+  RowCoord rc;
+  uint32_t rbk = fdiv(t, d.div_nseg);
+  uint32_t seg = t - rbk * d.nseg;
+  rc.row0 = rbk * d.RB;
+  rc.nr = umin32(d.RB, d.nrows - rc.row0);
+  rc.seg0 = seg * d.SEG;
+  rc.slen = umin32(d.SEG, d.L - rc.seg0);
+  return rc;
+}
+
+inline size_t row_lds_bytes(const RowDesc &d) { return sizeof(int64_t) * 2 * (size_t)d.RB; }
+
+HPTTX_HD void row_tables(const RowDesc &d, const RowCoord &rc, uint32_t tid,
+                         int64_t *tabA, int64_t *tabB) {
+  for (uint32_t i = tid; i < rc.nr; i += kBlock) {
+    int64_t a, b;
+    dl_offsets(d.W, rc.row0 + i, a, b);
+    tabA[i] = a;
+    tabB[i] = b;
+  }
+}
+
+template <typename T, typename S, int NC, int V, bool UPD, bool CC, bool NT>
+HPTTX_HD void row_body(const RowDesc &d, const RowCoord &rc, uint32_t tid,
+                       const T *HPTTX_RESTRICT A, T *HPTTX_RESTRICT B,
+                       const int64_t *tabA, const int64_t *tabB, const Coef<S> &k) {
+  const uint32_t nvs = d.nvs;
+  const uint32_t items = nvs * rc.nr;
+  for (uint32_t k0 = tid; k0 < items; k0 += kBlock * kUnroll) {
+    T x[kUnroll][V];
+    T y[kUnroll][V];
+    int64_t boff[kUnroll];
+    bool ok[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const uint32_t kk = k0 + (uint32_t)u * kBlock;
+      const uint32_t rr = fdiv(kk, d.div_nvs);
+      const uint32_t v = kk - rr * nvs;
+      ok[u] = (kk < items) && (v * V < rc.slen);
+      boff[u] = 0;
+      if (ok[u]) {
+        const int64_t e = (int64_t)rc.seg0 + (int64_t)v * V;
+        boff[u] = tabB[rr] + e * d.rb;
+        const int64_t aoff = tabA[rr] + e * d.ra;
+        vload<T, V>(A + aoff, x[u]);
+        if (UPD) vload<T, V>(B + boff[u], y[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if (!ok[u]) continue;
+      T out[V];
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        const Elem<S, NC> &xe = reinterpret_cast<const Elem<S, NC> &>(x[u][j]);
+        const Elem<S, NC> &ye = reinterpret_cast<const Elem<S, NC> &>(y[u][j]);
+        Elem<S, NC> re = axpby<S, NC, UPD, CC>(xe, ye, k);
+        __builtin_memcpy(&out[j], &re, sizeof(T));
+      }
+      vstore<T, V, NT>(B + boff[u], out);
+    }
+  }
+}
+
+}  // namespace hpttx

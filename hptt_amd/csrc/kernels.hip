@@ -1,0 +1,125 @@
+// kernels.hip -- gfx950 kernels for one transposition and their dispatch.
+//
+// TILE  (k_tile): LDS-staged transpose of an R x C x O block tile; coalesced
+//       vector reads along A's stride-1 super-dim, coalesced vector writes
+//       along B's.  Replaces the reference's macro/micro kernels
+//       (src/kernels/macro_kernel_trans.cc:50-142, src/arch/avx2/
+//       kernel_trans_avx2.cc:21-467) and its loop nest + OpenMP split
+//       (inc/hptt/operations/operation_trans.tcc:92-118,
+//       inc/hptt/cgraph/cgraph_trans.tcc:27-48).
+// ROWS  (k_rows): rows copied with alpha/beta; the stride-1-kept "wide copy"
+//       (reference MacroTransLinear, macro_kernel_trans.cc:148-195,
+//       kernel_trans_avx2.cc:480-530), the identity/axpby case and a strided
+//       gather for layouts without a unit-stride dim.
+// Non-temporal stores are used whenever B is not read (beta below threshold),
+// as the reference streams stores in that case (macro_kernel_trans.cc:60-62).
+#include <hip/hip_runtime.h>
+
+#include "hpttx_core.h"
+#include "kernels.h"
+
+namespace hpttx {
+
+template <typename T, typename S, int NC, int VR, int VW, bool UPD, bool CC, bool SH>
+__global__ __launch_bounds__(kBlock) void k_tile(TileDesc d, const T *__restrict__ A,
+                                                 T *__restrict__ B, Coef<S> co) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T *lds = reinterpret_cast<T *>(smem);
+  const TileTabs tb = tile_tabs(d, smem, sizeof(T));
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+    const TileCoord tc = tile_coord(d, t);
+    tile_tables(d, tc, tid, tb);
+    __syncthreads();
+    tile_read<T, VR>(d, tc, tid, A, lds, tb);
+    __syncthreads();
+    tile_write<T, S, NC, VW, UPD, CC, SH, !UPD>(d, tc, tid, B, lds, tb, co);
+    __syncthreads();
+  }
+}
+
+template <typename T, typename S, int NC, int V, bool UPD, bool CC>
+__global__ __launch_bounds__(kBlock) void k_rows(RowDesc d, const T *__restrict__ A,
+                                                 T *__restrict__ B, Coef<S> co) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int64_t *tabA = reinterpret_cast<int64_t *>(smem);
+  int64_t *tabB = tabA + d.RB;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+    const RowCoord rc = row_coord(d, t);
+    row_tables(d, rc, tid, tabA, tabB);
+    __syncthreads();
+    row_body<T, S, NC, V, UPD, CC, !UPD>(d, rc, tid, A, B, tabA, tabB, co);
+    __syncthreads();
+  }
+}
+
+namespace {
+
+template <typename T, typename S, int NC, int VCAP, bool UPD, bool CC>
+hipError_t launch_typed(const LaunchPlan &lp, const void *A, void *B,
+                        const Coef<S> &co, hipStream_t st) {
+  const T *a = static_cast<const T *>(A);
+  T *b = static_cast<T *>(B);
+  dim3 grid(lp.grid), block(kBlock);
+  if (lp.grid == 0) return hipSuccess;
+  if (lp.kind == kTile) {
+    const TileDesc &d = lp.tile;
+    const bool sh = d.VS > 1;
+    const bool vr = d.vr == VCAP && VCAP > 1;
+    const bool vw = d.vw == VCAP && VCAP > 1;
+#define HPTTX_TILE(VR_, VW_, SH_)                                                    \
+  hipLaunchKernelGGL((k_tile<T, S, NC, VR_, VW_, UPD, CC, SH_>), grid, block,      \
+                     lp.lds_bytes, st, d, a, b, co)
+    if (sh) {
+      if (vr && vw) HPTTX_TILE(VCAP, VCAP, true);
+      else if (vr) HPTTX_TILE(VCAP, 1, true);
+      else if (vw) HPTTX_TILE(1, VCAP, true);
+      else HPTTX_TILE(1, 1, true);
+    } else {
+      if (vr && vw) HPTTX_TILE(VCAP, VCAP, false);
+      else if (vr) HPTTX_TILE(VCAP, 1, false);
+      else if (vw) HPTTX_TILE(1, VCAP, false);
+      else HPTTX_TILE(1, 1, false);
+    }
+#undef HPTTX_TILE
+  } else {
+    const RowDesc &d = lp.rows;
+    if (d.v == VCAP && VCAP > 1)
+      hipLaunchKernelGGL((k_rows<T, S, NC, VCAP, UPD, CC>), grid, block, lp.lds_bytes,
+                         st, d, a, b, co);
+    else
+      hipLaunchKernelGGL((k_rows<T, S, NC, 1, UPD, CC>), grid, block, lp.lds_bytes, st,
+                         d, a, b, co);
+  }
+  return hipGetLastError();
+}
+
+template <typename T, typename S, int NC, int VCAP, bool CCOK>
+hipError_t launch_upd(const LaunchPlan &lp, const void *A, void *B, const CoefD &c,
+                      bool upd, bool cc, hipStream_t st) {
+  Coef<S> co{(S)c.ar, (S)c.ai, (S)c.br, (S)c.bi};
+  if (cc && CCOK) {
+    if (upd) return launch_typed<T, S, NC, VCAP, true, CCOK>(lp, A, B, co, st);
+    return launch_typed<T, S, NC, VCAP, false, CCOK>(lp, A, B, co, st);
+  }
+  if (upd) return launch_typed<T, S, NC, VCAP, true, false>(lp, A, B, co, st);
+  return launch_typed<T, S, NC, VCAP, false, false>(lp, A, B, co, st);
+}
+
+}  // namespace
+
+hipError_t launch_plan(int dtype, const LaunchPlan &lp, const void *A, void *B,
+                       const CoefD &c, bool update, bool cc, hipStream_t st) {
+  switch (dtype) {
+    case kF32: return launch_upd<Elem<float, 1>, float, 1, 4, false>(lp, A, B, c, update, false, st);
+    case kF64: return launch_upd<Elem<double, 1>, double, 1, 2, false>(lp, A, B, c, update, false, st);
+    case kC64: return launch_upd<Elem<float, 2>, float, 2, 2, true>(lp, A, B, c, update, cc, st);
+    case kC128: return launch_upd<Elem<double, 2>, double, 2, 1, true>(lp, A, B, c, update, cc, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+uint32_t vec_cap(int dtype) { return 16 / dtype_esize(dtype); }
+
+}  // namespace hpttx

@@ -1,0 +1,419 @@
+// planner.cpp -- see planner.h.
+#include "planner.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <sstream>
+
+namespace hpttx {
+
+uint32_t dtype_esize(int dtype) {
+  switch (dtype) {
+    case kF32: return 4;
+    case kF64: return 8;
+    case kC64: return 8;
+    case kC128: return 16;
+    default: return 0;
+  }
+}
+
+int build_problem(int dtype, uint32_t rank, const uint32_t *in_size,
+                  const uint32_t *perm, const uint32_t *in_outer,
+                  const uint32_t *out_outer, Problem &p, std::string &err) {
+  p = Problem();
+  p.dtype = dtype;
+  p.esize = dtype_esize(dtype);
+  if (p.esize == 0) { err = "unknown dtype"; return kEInval; }
+  // src/hptt.cc:30-33: order <= 1 -> nullptr.  Ranks above 6 are rejected
+  // rather than accepted and silently ignored (hptt_trans_gen.py:41-59).
+  if (rank <= 1) { err = "rank must be at least 2"; return kEInval; }
+  if (rank > (uint32_t)kMaxRank) { err = "rank above 6 is not supported"; return kEUnsupported; }
+  if (!in_size || !perm) { err = "null size or perm array"; return kEInval; }
+  // src/hptt.cc:49-56: permutation must be a bijection of [0, rank).
+  bool seen[kMaxRank] = {false};
+  for (uint32_t d = 0; d < rank; ++d) {
+    if (perm[d] >= rank || seen[perm[d]]) { err = "perm is not a permutation"; return kEInval; }
+    seen[perm[d]] = true;
+  }
+  // src/hptt.cc:41-47: zero sizes and outer < inner.  The output check uses
+  // out_outer[d] >= in_size[perm[d]] (B dim d holds A dim perm[d]).
+  for (uint32_t d = 0; d < rank; ++d) {
+    if (in_size[d] == 0) { err = "zero-sized dim"; return kEInval; }
+    if (in_outer && in_outer[d] < in_size[d]) { err = "in_outer_size smaller than in_size"; return kEInval; }
+    if (out_outer && out_outer[d] < in_size[perm[d]]) { err = "out_outer_size smaller than the output size"; return kEInval; }
+  }
+  p.rank = rank;
+  uint32_t out_size[kMaxRank];
+  for (uint32_t d = 0; d < rank; ++d) {
+    p.in_size[d] = in_size[d];
+    p.perm[d] = perm[d];
+    p.in_outer[d] = in_outer ? in_outer[d] : in_size[d];
+    out_size[d] = in_size[perm[d]];
+    p.out_outer[d] = out_outer ? out_outer[d] : out_size[d];
+  }
+  // Column-major strides over the outer sizes (inc/hptt/tensor.tcc:186-196).
+  int64_t sa[kMaxRank], sbB[kMaxRank];
+  sa[0] = 1;
+  sbB[0] = 1;
+  for (uint32_t d = 0; d + 1 < rank; ++d) {
+    sa[d + 1] = sa[d] * (int64_t)p.in_outer[d];
+    sbB[d + 1] = sbB[d] * (int64_t)p.out_outer[d];
+  }
+  uint64_t spanA = 1, spanB = 1, N = 1;
+  for (uint32_t d = 0; d < rank; ++d) {
+    spanA += (uint64_t)(in_size[d] - 1) * (uint64_t)sa[d];
+    spanB += (uint64_t)(out_size[d] - 1) * (uint64_t)sbB[d];
+    N *= in_size[d];
+  }
+  p.spanA = spanA;
+  p.spanB = spanB;
+  p.N = N;
+  // A dim i sits at B dim q with perm[q] = i.
+  int64_t sb_of_a[kMaxRank];
+  for (uint32_t q = 0; q < rank; ++q) sb_of_a[perm[q]] = sbB[q];
+  // Squeeze size-1 dims, then fuse neighbours that are contiguous in both A
+  // and B.  This is the reference's merge rule (parameter_trans.tcc:235-250:
+  // perm[k] == perm[k-1]+1 with unpadded inner dims) stated on strides.
+  int n = 0;
+  FusedDim tmp[kMaxRank];
+  for (uint32_t i = 0; i < rank; ++i) {
+    if (in_size[i] == 1) continue;
+    tmp[n++] = FusedDim{in_size[i], sa[i], sb_of_a[i]};
+  }
+  int m = 0;
+  for (int i = 0; i < n; ++i) {
+    if (m > 0) {
+      FusedDim &last = p.d[m - 1];
+      if (tmp[i].sa == last.sa * (int64_t)last.n && tmp[i].sb == last.sb * (int64_t)last.n) {
+        last.n *= tmp[i].n;
+        continue;
+      }
+    }
+    p.d[m++] = tmp[i];
+  }
+  p.fr = m;
+  for (int i = 0; i < m; ++i) p.border[i] = i;
+  std::sort(p.border, p.border + m, [&](int x, int y) { return p.d[x].sb < p.d[y].sb; });
+  return kOk;
+}
+
+uint32_t align_vec(const void *ptr, uint32_t esize, uint32_t cap) {
+  uintptr_t a = (uintptr_t)ptr;
+  uint32_t v = cap;
+  while (v > 1 && (a % (uintptr_t)(v * esize)) != 0) v >>= 1;
+  return v;
+}
+
+static void set_dimlist(DimList &L) {
+  // caller filled nd, div[i].d (as plain extents), sa, sb
+  for (uint32_t i = 0; i < L.nd; ++i) L.div[i] = make_fastdiv(L.div[i].d);
+  for (uint32_t i = L.nd; i < (uint32_t)kMaxRank; ++i) {
+    L.div[i] = make_fastdiv(1);
+    L.sa[i] = 0;
+    L.sb[i] = 0;
+  }
+}
+
+static bool divisible(int64_t x, uint32_t v) { return (x % (int64_t)v) == 0; }
+
+TuneParams default_tune(const Problem &p) {
+  TuneParams t;
+  // Runs of 256 B on each side and ~16 KiB of LDS per block tile.
+  t.run_r = std::max<uint32_t>(16, 256 / p.esize);
+  t.run_c = std::max<uint32_t>(16, 256 / p.esize);
+  t.block_elems = 16384 / p.esize;
+  t.wide_min_bytes = 256;
+  return t;
+}
+
+static void fill_tune_defaults(const Problem &p, TuneParams &t) {
+  TuneParams d = default_tune(p);
+  if (!t.run_r) t.run_r = d.run_r;
+  if (!t.run_c) t.run_c = d.run_c;
+  if (!t.block_elems) t.block_elems = d.block_elems;
+  if (!t.wide_min_bytes) t.wide_min_bytes = d.wide_min_bytes;
+}
+
+// ROWS plan over a row dim `rowdim` (index into p.d, or -1 for a single
+// element) and every other dim as the row index walked in B order.
+static int plan_rows(const Problem &p, int rowdim, const TuneParams &tp,
+                     LaunchPlan &lp, std::string &err) {
+  RowDesc &r = lp.rows;
+  r = RowDesc();
+  const uint32_t vcap = 16 / p.esize;
+  uint64_t L = rowdim >= 0 ? p.d[rowdim].n : 1;
+  r.ra = rowdim >= 0 ? p.d[rowdim].sa : 1;
+  r.rb = rowdim >= 0 ? p.d[rowdim].sb : 1;
+  uint64_t nrows = 1;
+  DimList &W = r.W;
+  W.nd = 0;
+  for (int k = 0; k < p.fr; ++k) {
+    int i = p.border[k];
+    if (i == rowdim) continue;
+    W.div[W.nd].d = (uint32_t)p.d[i].n;
+    W.sa[W.nd] = p.d[i].sa;
+    W.sb[W.nd] = p.d[i].sb;
+    W.nd++;
+    nrows *= p.d[i].n;
+  }
+  if (L > 0xFFFFFFFFull || nrows > 0xFFFFFFFFull) {
+    err = "row extent or row count exceeds 2^32";
+    return kEUnsupported;
+  }
+  set_dimlist(W);
+  r.L = (uint32_t)L;
+  r.nrows = (uint32_t)nrows;
+  // Vector width the shape allows: contiguous rows whose length and every
+  // row offset are multiples of v.
+  uint32_t v = 1;
+  if (r.ra == 1 && r.rb == 1) {
+    v = vcap;
+    while (v > 1) {
+      bool okv = (L % v) == 0;
+      for (uint32_t i = 0; i < W.nd && okv; ++i)
+        okv = divisible(W.sa[i], v) && divisible(W.sb[i], v);
+      if (okv) break;
+      v >>= 1;
+    }
+  }
+  lp.vr_shape = lp.vw_shape = v;
+  // A block tile covers ~block_elems elements: one segment of RB rows.
+  const uint32_t be = std::max<uint32_t>(tp.block_elems, kBlock);
+  uint32_t seg = (uint32_t)std::min<uint64_t>(L, be);
+  if (seg < L) seg = std::max<uint32_t>(v, seg - seg % v);
+  uint32_t rb = std::max<uint32_t>(1, be / std::max<uint32_t>(seg, 1));
+  rb = std::min<uint32_t>(rb, 1024);
+  rb = (uint32_t)std::min<uint64_t>(rb, nrows);
+  r.SEG = seg;
+  r.nseg = (uint32_t)((L + seg - 1) / seg);
+  r.RB = rb;
+  r.nrb = (uint32_t)((nrows + rb - 1) / rb);
+  uint64_t nt = (uint64_t)r.nseg * r.nrb;
+  if (nt > 0xFFFFFFFFull) { err = "too many tiles"; return kEUnsupported; }
+  r.ntiles = (uint32_t)nt;
+  r.div_nseg = make_fastdiv(r.nseg);
+  lp.kind = kRows;
+  lp.tune = tp;
+  if (rowdim < 0 || p.fr == 1)
+    lp.path = "axpby";
+  else if (r.ra == 1 && r.rb == 1)
+    lp.path = "wide-copy";
+  else
+    lp.path = "gather";
+  return kOk;
+}
+
+static int plan_tile(const Problem &p, bool shared, const TuneParams &tp,
+                     LaunchPlan &lp, std::string &err) {
+  TileDesc &t = lp.tile;
+  t = TileDesc();
+  const uint32_t vcap = 16 / p.esize;
+  const int a0 = 0;              // A's stride-1 dim
+  const int b0 = p.border[0];    // B's stride-1 dim
+  const uint32_t VS = shared ? (uint32_t)p.d[0].n : 1;
+  bool inR[kMaxRank] = {false}, inC[kMaxRank] = {false};
+  // R grows along A order from the first dim after S, C along B order.
+  int ra = shared ? 1 : a0;      // index into A order
+  int cb = shared ? 1 : 0;       // index into B order
+  if (ra >= p.fr || cb >= p.fr) { err = "internal: tile groups empty"; return kEInval; }
+  if (inC[ra]) { err = "internal"; return kEInval; }
+  std::vector<int> Rd{ra}, Cd{p.border[cb]};
+  inR[ra] = true;
+  inC[p.border[cb]] = true;
+  if (Rd[0] == Cd[0]) { err = "internal: R and C share a dim"; return kEInval; }
+  // contiguity of the first R dim after S in A, first C dim after S in B
+  if (shared) {
+    if (p.d[ra].sa != (int64_t)VS || p.d[p.border[cb]].sb != (int64_t)VS) {
+      err = "internal: shared group not contiguous";
+      return kEInval;
+    }
+  } else {
+    if (p.d[a0].sa != 1 || p.d[b0].sb != 1) { err = "internal: stride-1 dims"; return kEInval; }
+  }
+  uint64_t VR = p.d[ra].n, VC = p.d[p.border[cb]].n;
+  for (bool grew = true; grew;) {
+    grew = false;
+    if (VR * VS < tp.run_r && ra + 1 < p.fr) {
+      int nx = ra + 1;
+      const FusedDim &cur = p.d[ra];
+      if (!inC[nx] && p.d[nx].sa == cur.sa * (int64_t)cur.n) {
+        ra = nx;
+        inR[nx] = true;
+        Rd.push_back(nx);
+        VR *= p.d[nx].n;
+        grew = true;
+      }
+    }
+    if (VC * VS < tp.run_c && cb + 1 < p.fr) {
+      int nx = p.border[cb + 1];
+      const FusedDim &cur = p.d[p.border[cb]];
+      if (!inR[nx] && p.d[nx].sb == cur.sb * (int64_t)cur.n) {
+        cb = cb + 1;
+        inC[nx] = true;
+        Cd.push_back(nx);
+        VC *= p.d[nx].n;
+        grew = true;
+      }
+    }
+  }
+  std::vector<int> Od;
+  uint64_t VO = 1;
+  for (int i = 0; i < p.fr; ++i) {
+    if (inR[i] || inC[i] || (shared && i == 0)) continue;
+    Od.push_back(i);
+    VO *= p.d[i].n;
+  }
+  if (VR > 0xFFFFFFFFull || VC > 0xFFFFFFFFull || VO > 0xFFFFFFFFull) {
+    err = "a tile super-dim exceeds 2^32 indices";
+    return kEUnsupported;
+  }
+  t.VS = VS;
+  t.VR = (uint32_t)VR;
+  t.VC = (uint32_t)VC;
+  t.VO = (uint32_t)VO;
+  auto fill = [&](DimList &L, const std::vector<int> &dims) {
+    L.nd = (uint32_t)dims.size();
+    for (size_t k = 0; k < dims.size(); ++k) {
+      L.div[k].d = (uint32_t)p.d[dims[k]].n;
+      L.sa[k] = p.d[dims[k]].sa;
+      L.sb[k] = p.d[dims[k]].sb;
+    }
+    set_dimlist(L);
+  };
+  fill(t.R, Rd);
+  fill(t.C, Cd);
+  fill(t.O, Od);
+  // Vector widths the shape allows.  Reads: runs r0*VS + [0, rlen*VS) at A
+  // offsets from C and O; writes: runs at B offsets from R and O.
+  uint32_t vr = vcap, vw = vcap;
+  auto ok_r = [&](uint32_t v) {
+    if (((uint64_t)VS * VR) % v) return false;
+    for (int i : Cd) if (!divisible(p.d[i].sa, v)) return false;
+    for (int i : Od) if (!divisible(p.d[i].sa, v)) return false;
+    return true;
+  };
+  auto ok_w = [&](uint32_t v) {
+    if (((uint64_t)VS * VC) % v) return false;
+    for (int i : Rd) if (!divisible(p.d[i].sb, v)) return false;
+    for (int i : Od) if (!divisible(p.d[i].sb, v)) return false;
+    return true;
+  };
+  while (vr > 1 && !ok_r(vr)) vr >>= 1;
+  while (vw > 1 && !ok_w(vw)) vw >>= 1;
+  // Tile extents: TR*VS ~ run_r, TC*VS ~ run_c, multiples of the vectors.
+  uint32_t TR = (uint32_t)std::min<uint64_t>(VR, std::max<uint32_t>(1, tp.run_r / VS));
+  uint32_t TC = (uint32_t)std::min<uint64_t>(VC, std::max<uint32_t>(1, tp.run_c / VS));
+  // Whole-tile runs must stay multiples of the vector widths.
+  while (vr > 1 && ((uint64_t)TR * VS) % vr) vr >>= 1;
+  while (vw > 1 && ((uint64_t)TC * VS) % vw) vw >>= 1;
+  uint64_t per_o = (uint64_t)TR * VS * TC;
+  uint32_t TO = (uint32_t)std::max<uint64_t>(1, tp.block_elems / std::max<uint64_t>(per_o, 1));
+  TO = (uint32_t)std::min<uint64_t>(TO, VO);
+  TO = std::min<uint32_t>(TO, 512);
+  t.TR = TR;
+  t.TC = TC;
+  t.TO = TO;
+  t.nR = (uint32_t)((VR + TR - 1) / TR);
+  t.nC = (uint32_t)((VC + TC - 1) / TC);
+  t.nO = (uint32_t)((VO + TO - 1) / TO);
+  uint64_t nt = (uint64_t)t.nR * t.nC * t.nO;
+  if (nt > 0xFFFFFFFFull) { err = "too many tiles"; return kEUnsupported; }
+  t.ntiles = (uint32_t)nt;
+  t.div_nR = make_fastdiv(t.nR);
+  t.div_nC = make_fastdiv(t.nC);
+  t.div_TC = make_fastdiv(TC);
+  t.div_TR = make_fastdiv(TR);
+  t.div_VS = make_fastdiv(VS);
+  lp.vr_shape = vr;
+  lp.vw_shape = vw;
+  lp.kind = kTile;
+  lp.tune = tp;
+  lp.path = shared ? "tile-shared" : "tile";
+  return kOk;
+}
+
+void finalize_vectors(const Problem &p, LaunchPlan &lp, uint32_t vr, uint32_t vw) {
+  if (lp.kind == kTile) {
+    TileDesc &t = lp.tile;
+    t.vr = vr;
+    t.vw = vw;
+    t.nvr = (t.TR * t.VS) / vr;
+    t.nvw = (t.TC * t.VS) / vw;
+    t.div_nvr = make_fastdiv(t.nvr);
+    t.div_nvw = make_fastdiv(t.nvw);
+    // LDS rows: a multiple of the read vector, padded by one vector so that
+    // the write pass's column gathers spread over banks.
+    uint32_t row = t.TR * t.VS;
+    row = (row + vr - 1) / vr * vr;
+    t.lds_row = row + vr;
+    if (vr == 1 && (row % 2) == 0) t.lds_row = row + 1;
+    lp.lds_bytes = tile_lds_bytes(t, p.esize);
+    lp.grid = std::min<uint32_t>(t.ntiles, kMaxGrid);
+  } else {
+    RowDesc &r = lp.rows;
+    r.v = vr;
+    r.nvs = (r.SEG + vr - 1) / vr;
+    r.div_nvs = make_fastdiv(r.nvs);
+    lp.lds_bytes = row_lds_bytes(r);
+    lp.grid = std::min<uint32_t>(r.ntiles, kMaxGrid);
+  }
+  (void)vw;
+}
+
+int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
+                std::string &err) {
+  TuneParams tp = tp_in;
+  fill_tune_defaults(p, tp);
+  lp = LaunchPlan();
+  int rc;
+  if (p.fr == 0) {
+    rc = plan_rows(p, -1, tp, lp, err);            // a single element
+  } else if (p.fr == 1) {
+    rc = plan_rows(p, 0, tp, lp, err);             // identity: axpby
+  } else if (p.d[0].sa != 1 || p.d[p.border[0]].sb != 1) {
+    rc = plan_rows(p, p.border[0], tp, lp, err);   // no unit-stride dim (padded size-1 dims)
+  } else if (p.border[0] == 0) {
+    // Stride-1 dim kept (reference "common leading",
+    // parameter_trans.tcc:154-158).  Long rows: wide copy; short rows: TILE
+    // with the rows as a shared innermost group.
+    const uint64_t L = p.d[0].n;
+    const bool contiguous_after = p.d[1].sa == (int64_t)L && p.d[p.border[1]].sb == (int64_t)L;
+    if (L * p.esize >= tp.wide_min_bytes || !contiguous_after || p.border[1] == 1)
+      rc = plan_rows(p, 0, tp, lp, err);
+    else
+      rc = plan_tile(p, true, tp, lp, err);
+  } else {
+    rc = plan_tile(p, false, tp, lp, err);
+  }
+  if (rc != kOk) return rc;
+  finalize_vectors(p, lp, lp.vr_shape, lp.vw_shape);
+  if (lp.lds_bytes > 65536) { err = "tile needs more than 64 KiB of LDS"; return kEUnsupported; }
+  return kOk;
+}
+
+std::string describe(const Problem &p, const LaunchPlan &lp) {
+  std::ostringstream os;
+  static const char *names[] = {"float", "double", "complex<float>", "complex<double>"};
+  os << "dtype: " << names[p.dtype] << "  rank: " << p.rank << "  elements: " << p.N << "\n";
+  os << "fused dims (A order, size:strideA:strideB):";
+  for (int i = 0; i < p.fr; ++i) os << " " << p.d[i].n << ":" << p.d[i].sa << ":" << p.d[i].sb;
+  os << "\nB order:";
+  for (int i = 0; i < p.fr; ++i) os << " " << p.border[i];
+  os << "\npath: " << lp.path;
+  if (lp.kind == kTile) {
+    const TileDesc &t = lp.tile;
+    os << "\ntile: VS=" << t.VS << " R=" << t.VR << "/" << t.TR << " C=" << t.VC << "/" << t.TC
+       << " O=" << t.VO << "/" << t.TO << " vec(r,w)=" << t.vr << "," << t.vw
+       << " tiles=" << t.ntiles << " lds=" << lp.lds_bytes << "B";
+  } else {
+    const RowDesc &r = lp.rows;
+    os << "\nrows: L=" << r.L << " stride(a,b)=" << r.ra << "," << r.rb << " rows=" << r.nrows
+       << " seg=" << r.SEG << " rows/block=" << r.RB << " vec=" << r.v << " tiles=" << r.ntiles;
+  }
+  os << "\ngrid: " << lp.grid << " x " << kBlock << "  tune(run_r,run_c,block,wide)="
+     << lp.tune.run_r << "," << lp.tune.run_c << "," << lp.tune.block_elems << ","
+     << lp.tune.wide_min_bytes << "\n";
+  return os.str();
+}
+
+}  // namespace hpttx

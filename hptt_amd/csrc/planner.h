@@ -1,0 +1,99 @@
+// planner.h -- host-side problem normalisation and launch planning.
+//
+// Replaces the reference's problem model and planner
+// (inc/hptt/param/parameter_trans.tcc:103-283 dimension merging and leading-dim
+// strides; inc/hptt/plan/plan_trans_util.tcc loop-order / thread heuristics)
+// with: dimension fusion on strides, a path choice (TILE / ROWS), the
+// TILE super-dim groups and tile shape, and vector widths.
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+
+#include "hpttx_core.h"
+
+namespace hpttx {
+
+// Tile-shape knobs the autotuner searches (0 = model default).
+struct TuneParams {
+  uint32_t run_r = 0;        // target read-run length along A, in elements
+  uint32_t run_c = 0;        // target write-run length along B, in elements
+  uint32_t block_elems = 0;  // target elements per block tile
+  uint32_t wide_min_bytes = 0;  // stride-1-kept rows at least this long use ROWS
+  bool operator==(const TuneParams &o) const {
+    return run_r == o.run_r && run_c == o.run_c && block_elems == o.block_elems &&
+           wide_min_bytes == o.wide_min_bytes;
+  }
+};
+
+struct FusedDim {
+  uint64_t n;
+  int64_t sa, sb;
+};
+
+// The problem after validation, size-1 squeeze and fusion.  Dims are kept in
+// A order (ascending sa); border lists them in B order (ascending sb).
+struct Problem {
+  int dtype = 0;
+  uint32_t esize = 4;
+  uint32_t rank = 0;          // user rank
+  uint32_t in_size[kMaxRank] = {0};
+  uint32_t perm[kMaxRank] = {0};
+  uint32_t in_outer[kMaxRank] = {0};
+  uint32_t out_outer[kMaxRank] = {0};
+  int fr = 0;                 // fused rank
+  FusedDim d[kMaxRank];
+  int border[kMaxRank];
+  uint64_t N = 0;             // elements moved
+  uint64_t spanA = 0, spanB = 0;  // elements each buffer must hold
+};
+
+enum Kind : int { kTile = 0, kRows = 1 };
+
+struct LaunchPlan {
+  Kind kind = kRows;
+  TileDesc tile{};
+  RowDesc rows{};
+  uint32_t vr_shape = 1, vw_shape = 1;  // widest vectors the strides allow
+  uint32_t grid = 0;
+  size_t lds_bytes = 0;
+  TuneParams tune{};
+  std::string path;            // "tile", "tile-shared", "wide-copy", "axpby", "gather"
+};
+
+// Status codes shared with the C ABI.
+enum Status : int {
+  kOk = 0,
+  kEInval = 1,
+  kENoMem = 2,
+  kEHip = 3,
+  kEUnsupported = 4,
+};
+
+uint32_t dtype_esize(int dtype);
+
+// Validation (src/hptt.cc:25-56) + normalisation.  On error, `err` says why.
+int build_problem(int dtype, uint32_t rank, const uint32_t *in_size,
+                  const uint32_t *perm, const uint32_t *in_outer,
+                  const uint32_t *out_outer, Problem &p, std::string &err);
+
+// Shape plan for the widest legal vectors (capped by max_vec_bytes).
+int plan_launch(const Problem &p, const TuneParams &tp, LaunchPlan &lp,
+                std::string &err);
+
+// Re-derive the vector-dependent fields of lp for vector widths (vr, vw);
+// both must divide what the shape allows.
+void finalize_vectors(const Problem &p, LaunchPlan &lp, uint32_t vr, uint32_t vw);
+
+// Widest vector (elements) a pointer's alignment allows for this element size.
+uint32_t align_vec(const void *ptr, uint32_t esize, uint32_t cap);
+
+// Maximum number of blocks one launch uses (larger tile counts loop).
+constexpr uint32_t kMaxGrid = 1u << 20;
+
+std::string describe(const Problem &p, const LaunchPlan &lp);
+
+TuneParams default_tune(const Problem &p);
+
+}  // namespace hpttx

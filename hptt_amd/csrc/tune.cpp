@@ -1,0 +1,163 @@
+// tune.cpp -- see tune.h.
+#include "tune.h"
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+
+namespace hpttx {
+
+namespace {
+
+std::mutex g_mu;
+std::map<std::string, TuneParams> g_cache;
+bool g_file_loaded = false;
+
+const char *cache_path() {
+  const char *p = std::getenv("HPTTX_TUNE_CACHE");
+  return (p && *p) ? p : nullptr;
+}
+
+void load_file_locked() {
+  if (g_file_loaded) return;
+  g_file_loaded = true;
+  const char *path = cache_path();
+  if (!path) return;
+  std::ifstream f(path);
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    std::string key;
+    TuneParams t;
+    if (std::getline(is, key, '\t') && (is >> t.run_r >> t.run_c >> t.block_elems >> t.wide_min_bytes))
+      g_cache[key] = t;
+  }
+}
+
+void store_locked(const std::string &key, const TuneParams &t) {
+  g_cache[key] = t;
+  const char *path = cache_path();
+  if (!path) return;
+  std::ofstream f(path, std::ios::app);
+  if (f) f << key << '\t' << t.run_r << ' ' << t.run_c << ' ' << t.block_elems << ' '
+           << t.wide_min_bytes << '\n';
+}
+
+bool same_launch(const LaunchPlan &a, const LaunchPlan &b) {
+  if (a.kind != b.kind) return false;
+  if (a.kind == kTile)
+    return a.tile.TR == b.tile.TR && a.tile.TC == b.tile.TC && a.tile.TO == b.tile.TO &&
+           a.tile.VR == b.tile.VR && a.tile.VC == b.tile.VC && a.tile.VS == b.tile.VS;
+  return a.rows.SEG == b.rows.SEG && a.rows.RB == b.rows.RB && a.rows.L == b.rows.L;
+}
+
+}  // namespace
+
+std::string tune_key(const Problem &p, bool update) {
+  std::ostringstream os;
+  os << "gfx950|" << p.dtype << '|' << (update ? 1 : 0) << '|' << p.fr;
+  for (int i = 0; i < p.fr; ++i) os << '|' << p.d[i].n << ':' << p.d[i].sa << ':' << p.d[i].sb;
+  return os.str();
+}
+
+bool tune_lookup(const Problem &p, bool update, TuneParams &out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  load_file_locked();
+  auto it = g_cache.find(tune_key(p, update));
+  if (it == g_cache.end()) return false;
+  out = it->second;
+  return true;
+}
+
+std::vector<TuneParams> tune_candidates(const Problem &p) {
+  std::vector<TuneParams> c;
+  const TuneParams d = default_tune(p);
+  c.push_back(d);
+  const uint32_t e = p.esize;
+  const uint32_t runs[] = {64 / e, 128 / e, 256 / e, 512 / e};
+  const uint32_t blocks[] = {8192 / e, 16384 / e, 32768 / e};
+  const uint32_t wides[] = {128, 256, 1024};
+  for (uint32_t rr : runs)
+    for (uint32_t rc : runs)
+      for (uint32_t be : blocks)
+        for (uint32_t w : wides) {
+          TuneParams t{std::max<uint32_t>(rr, 4), std::max<uint32_t>(rc, 4), be, w};
+          if (!(t == d)) c.push_back(t);
+        }
+  return c;
+}
+
+int autotune(const Problem &p, const CoefD &coef, bool update, bool cc, double timeout_s,
+             hipStream_t stream, TuneParams &out, std::string &err) {
+  out = default_tune(p);
+  const size_t bytesA = p.spanA * p.esize, bytesB = p.spanB * p.esize;
+  void *A = nullptr, *B = nullptr;
+  if (hipMalloc(&A, bytesA) != hipSuccess) {
+    (void)hipGetLastError();
+    return kOk;  // no room for scratch: keep the model pick
+  }
+  if (hipMalloc(&B, bytesB) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(A);
+    return kOk;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = kOk;
+  if (hipMemsetAsync(A, 0, bytesA, stream) != hipSuccess ||
+      hipMemsetAsync(B, 0, bytesB, stream) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    err = "autotune: HIP setup failed";
+    rc = kEHip;
+  }
+  const auto t_start = std::chrono::steady_clock::now();
+  float best = 1e30f;
+  std::vector<LaunchPlan> seen;
+  if (rc == kOk) {
+    for (const TuneParams &t : tune_candidates(p)) {
+      if (timeout_s > 0) {
+        double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        if (el > timeout_s) break;
+      }
+      LaunchPlan lp;
+      std::string e2;
+      if (plan_launch(p, t, lp, e2) != kOk) continue;
+      bool dup = false;
+      for (const LaunchPlan &s : seen) dup = dup || same_launch(s, lp);
+      if (dup) continue;
+      seen.push_back(lp);
+      if (launch_plan(p.dtype, lp, A, B, coef, update, cc, stream) != hipSuccess) continue;
+      const int reps = 3;
+      (void)hipEventRecord(e0, stream);
+      for (int i = 0; i < reps; ++i) (void)launch_plan(p.dtype, lp, A, B, coef, update, cc, stream);
+      (void)hipEventRecord(e1, stream);
+      if (hipEventSynchronize(e1) != hipSuccess) {
+        err = "autotune: event sync failed";
+        rc = kEHip;
+        break;
+      }
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) {
+        best = ms;
+        out = t;
+      }
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(stream);
+  (void)hipFree(A);
+  (void)hipFree(B);
+  if (rc == kOk) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    load_file_locked();
+    store_locked(tune_key(p, update), out);
+  }
+  return rc;
+}
+
+}  // namespace hpttx

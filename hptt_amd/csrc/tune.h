@@ -1,0 +1,37 @@
+// tune.h -- tile/schedule autotuner with an in-process and on-disk cache.
+//
+// Replaces the reference's timed tuning over loop orders and thread splits
+// (inc/hptt/plan/plan_trans.tcc:33-67, candidates from
+// plan_trans_util.tcc:1183-1266) keeping its tuning_timeout meaning
+// (inc/hptt/hptt.h:46-57): 0 = model pick, > 0 = time-boxed, < 0 = every
+// candidate.  Unlike the reference, which times candidates on the caller's
+// buffers with alpha=0, beta=1 (plan_trans.tcc:42-44), tuning runs on scratch
+// device buffers and never touches the caller's B.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "planner.h"
+
+namespace hpttx {
+
+// Cache key: dtype, update flag and the fused problem (extents and strides).
+std::string tune_key(const Problem &p, bool update);
+
+// Look the problem up in the cache (memory, then the file named by the
+// HPTTX_TUNE_CACHE environment variable).
+bool tune_lookup(const Problem &p, bool update, TuneParams &out);
+
+// Time candidate tile shapes and keep the fastest; records it in the cache.
+int autotune(const Problem &p, const CoefD &coef, bool update, bool cc,
+             double timeout_s, hipStream_t stream, TuneParams &out,
+             std::string &err);
+
+// The candidate list the tuner walks (model default first).
+std::vector<TuneParams> tune_candidates(const Problem &p);
+
+}  // namespace hpttx

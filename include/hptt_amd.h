@@ -1,0 +1,159 @@
+/*
+ * hptt_amd.h -- C ABI of the MI355X (gfx950) HPTT transposition library.
+ *
+ *   B[perm(i)] = alpha * A[i] + beta * B[perm(i)]
+ *
+ * Column-major tensors (dim 0 is stride-1), ranks 2..6, element types float,
+ * double, complex<float> and complex<double> (interleaved re/im).  A and B are
+ * DEVICE pointers; exec enqueues on a HIP stream and returns without waiting.
+ *
+ * Every entry point returns an hpttx status code; hpttx_last_error() gives the
+ * reason for the last failure on the calling thread.  Plans are immutable
+ * after creation except for hpttx_plan_reset_data / hpttx_plan_set_stream;
+ * exec of one plan from several host threads is safe (no per-exec host state).
+ *
+ * Each declaration names the reference interface it replaces
+ * (tongsucn/hptt, file:line).
+ */
+#ifndef HPTT_AMD_H_
+#define HPTT_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HPTTX_VERSION 100
+
+/* Status codes.  The reference reports failure only as a null plan
+ * (inc/hptt/hptt.h:71-83); these refine that null. */
+#define HPTTX_OK 0
+#define HPTTX_EINVAL 1       /* invalid argument (src/hptt.cc:25-56 cases) */
+#define HPTTX_ENOMEM 2       /* host or device allocation failed */
+#define HPTTX_EHIP 3         /* a HIP runtime call failed */
+#define HPTTX_EUNSUPPORTED 4 /* rank > 6 or extents beyond the kernel limits */
+
+/* dtype codes used by the batch and serving interfaces */
+#define HPTTX_F32 0
+#define HPTTX_F64 1
+#define HPTTX_C64 2
+#define HPTTX_C128 3
+
+typedef struct hpttx_plan hpttx_plan;
+
+/*
+ * Plan creation.  Replaces the C++-mangled per-dtype factories
+ * create_trans_plan_impl_{s,d,c,z} (src/impl/hptt_trans_impl_s.cc:8-20, and
+ * the d/c/z twins) that hptt::create_plan<T> dlsyms (src/hptt.cc:66-92), with
+ * the validation of src/hptt.cc:25-56 folded in.
+ *
+ *   in, out          device pointers to A and B (non-owning; see reset_data)
+ *   rank             tensor order, 2..6 (the reference takes it from perm.size())
+ *   in_size[rank]    A's extents
+ *   perm[rank]       B dim d holds A dim perm[d]; out_size[d] = in_size[perm[d]]
+ *   alpha, beta      scaling; B is read iff fabs(beta) >= 1e-16
+ *                    (inc/hptt/impl/hptt_trans_impl.h:75-78)
+ *   num_threads      accepted and ignored (the reference's OpenMP thread count)
+ *   tuning_timeout   0: model pick; > 0: autotune at most this many seconds;
+ *                    < 0: exhaustive autotune (inc/hptt/hptt.h:46-57).
+ *                    Tuning runs on scratch buffers, never on `out`.
+ *   in_outer_size    NULL or [rank]: A's padded extents (strides)
+ *   out_outer_size   NULL or [rank]: B's padded extents
+ *   stream           hipStream_t the plan enqueues on (NULL = null stream)
+ *   plan             receives the plan; *plan = NULL on failure
+ */
+int hpttx_plan_create_s(const float *in, float *out, uint32_t rank,
+                        const uint32_t *in_size, const uint32_t *perm,
+                        float alpha, float beta, uint32_t num_threads,
+                        double tuning_timeout, const uint32_t *in_outer_size,
+                        const uint32_t *out_outer_size, void *stream,
+                        hpttx_plan **plan);
+int hpttx_plan_create_d(const double *in, double *out, uint32_t rank,
+                        const uint32_t *in_size, const uint32_t *perm,
+                        double alpha, double beta, uint32_t num_threads,
+                        double tuning_timeout, const uint32_t *in_outer_size,
+                        const uint32_t *out_outer_size, void *stream,
+                        hpttx_plan **plan);
+/* complex<float>; alpha/beta real, applied to re and im alike
+ * (inc/hptt/types.h:29-54) -- replaces create_trans_plan_impl_c */
+int hpttx_plan_create_c(const void *in, void *out, uint32_t rank,
+                        const uint32_t *in_size, const uint32_t *perm,
+                        float alpha, float beta, uint32_t num_threads,
+                        double tuning_timeout, const uint32_t *in_outer_size,
+                        const uint32_t *out_outer_size, void *stream,
+                        hpttx_plan **plan);
+/* complex<double>; replaces create_trans_plan_impl_z */
+int hpttx_plan_create_z(const void *in, void *out, uint32_t rank,
+                        const uint32_t *in_size, const uint32_t *perm,
+                        double alpha, double beta, uint32_t num_threads,
+                        double tuning_timeout, const uint32_t *in_outer_size,
+                        const uint32_t *out_outer_size, void *stream,
+                        hpttx_plan **plan);
+/* Complex coefficients (an extension: the reference's coefficients are real,
+ * inc/hptt/types.h:29-54, inc/hptt/hptt.h:89-90).  Order:
+ *   a*x = (fl(fl(ar*xr) - fl(ai*xi)), fl(fl(ar*xi) + fl(ai*xr))), b*y alike,
+ *   r = (fl(by.re + ax.re), fl(by.im + ax.im)); B read iff |br| or |bi| >= 1e-16.
+ * With ai == bi == 0 these dispatch to the real-coefficient plans above. */
+int hpttx_plan_create_c_cplx(const void *in, void *out, uint32_t rank,
+                             const uint32_t *in_size, const uint32_t *perm,
+                             float alpha_re, float alpha_im, float beta_re,
+                             float beta_im, uint32_t num_threads,
+                             double tuning_timeout,
+                             const uint32_t *in_outer_size,
+                             const uint32_t *out_outer_size, void *stream,
+                             hpttx_plan **plan);
+int hpttx_plan_create_z_cplx(const void *in, void *out, uint32_t rank,
+                             const uint32_t *in_size, const uint32_t *perm,
+                             double alpha_re, double alpha_im, double beta_re,
+                             double beta_im, uint32_t num_threads,
+                             double tuning_timeout,
+                             const uint32_t *in_outer_size,
+                             const uint32_t *out_outer_size, void *stream,
+                             hpttx_plan **plan);
+
+/* Enqueue the transposition on the plan's stream.  Replaces
+ * CGraphTransPackBase::exec / operator() (inc/hptt/impl/hptt_trans.h:27-28),
+ * which run synchronously; here B is final once the stream reaches this point
+ * (hpttx_plan_sync). */
+int hpttx_plan_exec(hpttx_plan *plan);
+/* Same, on another stream (does not change the plan's stream). */
+int hpttx_plan_exec_on(hpttx_plan *plan, void *stream);
+/* Wait for the plan's stream. */
+int hpttx_plan_sync(hpttx_plan *plan);
+/* Swap the data pointers; sizes, perm, alpha and beta stay fixed.  Replaces
+ * CGraphTransPackBase::reset_data (inc/hptt/impl/hptt_trans.h:30). */
+int hpttx_plan_reset_data(hpttx_plan *plan, const void *in, void *out);
+int hpttx_plan_set_stream(hpttx_plan *plan, void *stream);
+/* Human-readable plan (fused dims, path, tile, grid) into buf (NUL-terminated,
+ * truncated to len).  Returns the full length needed.  Replaces
+ * CGraphTransPackBase::print_plan (inc/hptt/impl/hptt_trans.h:29), which
+ * prints loop order and thread split. */
+int hpttx_plan_describe(const hpttx_plan *plan, char *buf, size_t len);
+
+typedef struct hpttx_plan_info {
+  uint64_t elements;        /* N */
+  uint32_t elem_bytes;      /* 4, 8, 8, 16 */
+  int32_t update;           /* 1 iff B is read */
+  int32_t path;             /* 0 tile, 1 tile-shared, 2 wide-copy, 3 axpby, 4 gather */
+  int32_t fused_rank;
+  uint64_t algorithmic_bytes; /* N*elem*(2 + update) */
+  uint32_t grid;            /* blocks per launch */
+  uint32_t lds_bytes;       /* LDS per block */
+  uint32_t tile_r, tile_c, tile_o, vec_r, vec_w;
+} hpttx_plan_info;
+int hpttx_plan_get_info(const hpttx_plan *plan, hpttx_plan_info *info);
+
+/* Release the plan (the reference's shared_ptr deleter, src/hptt.cc:89-92). */
+void hpttx_plan_destroy(hpttx_plan *plan);
+
+/* Reason for the last failure on this thread ("" if none). */
+const char *hpttx_last_error(void);
+int hpttx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPTT_AMD_H_ */

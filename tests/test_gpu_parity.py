@@ -1,0 +1,155 @@
+"""GPU parity: the HIP kernels through the C ABI against the oracle and the
+golden vectors, bit for bit (pure data movement plus the reference's
+two-rounding axpby, contraction off -- the tolerance is zero ulp)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import hptt_amd
+import oracle
+import shapes
+from gpu_util import dev, host, run_plan
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _case(c):
+    al = complex(*c["alpha"])
+    be = complex(*c["beta"])
+    if not c["complex_coef"]:
+        al, be = al.real, be.real
+    return al, be
+
+
+def test_golden_vectors(device):
+    man = json.load(open(os.path.join(HERE, "golden", "golden_small.json")))["cases"]
+    data = np.load(os.path.join(HERE, "golden", "golden_small.npz"))
+    for c in man:
+        al, be = _case(c)
+        A, B = data[c["name"] + "_A"], data[c["name"] + "_B"]
+        got, _ = run_plan(A, B, c["in_size"], c["perm"], al, be, c["in_outer"],
+                          c["out_outer"], c["complex_coef"])
+        assert oracle.first_mismatch(got, data[c["name"] + "_out"]) == -1, c["name"]
+
+
+def _vs_oracle(dt, size, perm, al, be, io=None, oo=None, cc=False, offset=0, tt=0.0,
+               seed=0):
+    na = int(np.prod(io if io else size))
+    nb = int(np.prod(oo if oo else [size[p] for p in perm]))
+    A = oracle.uniform(na, dt, 1000 + seed)
+    B0 = oracle.uniform(nb, dt, 2000 + seed)
+    exp = B0.copy()
+    assert oracle.transpose(A, exp, size, perm, al, be, io, oo, nthreads=16,
+                            complex_coef=cc) == 0
+    got, info = run_plan(A, B0, size, perm, al, be, io, oo, cc, offset, tt)
+    bad = oracle.first_mismatch(got, exp)
+    assert bad == -1, (np.dtype(dt).name, size, perm, al, be, bad)
+    return info
+
+
+@pytest.mark.parametrize("dt", shapes.DTYPES)
+def test_random_shapes(device, dt):
+    rng = np.random.default_rng(99 + shapes.DTYPES.index(dt))
+    for k in range(40):
+        size, perm = shapes.random_case(rng, max_elems=200000, max_dim=64)
+        al, be = [(1.0, 0.0), (2.3, 4.2), (-0.5, 1e-17), (1.25, -0.75)][k % 4]
+        _vs_oracle(dt, size, perm, al, be, seed=k)
+
+
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
+def test_complex_coefficients(device, dt):
+    rng = np.random.default_rng(5)
+    for k in range(16):
+        size, perm = shapes.random_case(rng, max_elems=100000, max_dim=64)
+        be = complex(0.7, -0.2) if k % 2 else 0j
+        _vs_oracle(dt, size, perm, complex(1.1, 0.3), be, cc=True, seed=k)
+
+
+@pytest.mark.parametrize("dt", shapes.DTYPES)
+def test_padded_and_misaligned(device, dt):
+    rng = np.random.default_rng(7)
+    for k in range(12):
+        size, perm = shapes.random_case(rng, max_elems=50000, max_dim=32)
+        io, oo = shapes.random_outer(rng, size, perm)
+        _vs_oracle(dt, size, perm, 2.3, 4.2 if k % 2 else 0.0, io, oo, seed=k)
+    for k in range(8):
+        size, perm = shapes.random_case(rng, max_elems=50000, max_dim=64)
+        _vs_oracle(dt, size, perm, 1.5, 0.25, offset=1, seed=k)
+
+
+def test_benchmark_workloads_full_size(device):
+    """The benchmark workloads at their full sizes, against the oracle."""
+    info = _vs_oracle(np.float32, *shapes.CONFIG1, 1.0, 0.0)
+    assert hptt_amd.PATHS[info.path] == "tile"
+    info = _vs_oracle(np.float32, *shapes.CONFIG2, 2.3, 4.2)
+    assert info.update == 1 and info.vec_r == 4 and info.vec_w == 4
+    _vs_oracle(np.float64, *shapes.CONFIG2, 2.3, 4.2)
+
+
+def test_complex_workloads_full_size(device):
+    for perm in (shapes.CONFIG3_TILE[1], shapes.CONFIG3_WIDE[1]):
+        _vs_oracle(np.complex128, shapes.CONFIG3_TILE[0], perm, complex(1.1, 0.3),
+                   complex(0.7, -0.2), cc=True)
+        # the real-coefficient twin the reference can express
+        _vs_oracle(np.complex128, shapes.CONFIG3_TILE[0], perm, 1.1, 0.7)
+
+
+def test_round_trip_and_write_once(device):
+    """Size-independent properties at a full benchmark size: perm then inverse
+    restores A exactly; with beta = 0 every element of B is overwritten (B
+    starts as NaN) and the result is a permutation of A's bits."""
+    import torch
+    size, perm = shapes.CONFIG2
+    n = int(np.prod(size))
+    g = torch.Generator(device="cuda").manual_seed(3)
+    A = torch.rand(n, device="cuda", generator=g, dtype=torch.float32) * 2 - 1
+    B = torch.full((n,), float("nan"), device="cuda", dtype=torch.float32)
+    C = torch.empty_like(A)
+    p1 = hptt_amd.create_plan_checked(A, B, size, perm, 1.0, 0.0)
+    inv = list(np.argsort(perm))
+    p2 = hptt_amd.create_plan_checked(B, C, [size[p] for p in perm], inv, 1.0, 0.0)
+    p1.exec()
+    p2.exec()
+    torch.cuda.synchronize()
+    assert not torch.isnan(B).any().item()
+    assert torch.equal(C.view(torch.int32), A.view(torch.int32))
+    assert torch.equal(torch.sort(B.view(torch.int32))[0], torch.sort(A.view(torch.int32))[0])
+
+
+def test_plan_interface(device):
+    """reset_data, exec on another stream, autotuning, describe."""
+    import torch
+    size, perm = [40, 33, 20], [2, 0, 1]
+    n = int(np.prod(size))
+    A1 = oracle.uniform(n, np.float64, 1)
+    A2 = oracle.uniform(n, np.float64, 2)
+    dA1, dA2 = dev(A1), dev(A2)
+    dB1, dB2 = torch.zeros(n, dtype=torch.float64, device="cuda"), torch.zeros(
+        n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    plan = hptt_amd.create_plan_checked(dA1, dB1, size, perm, 2.0, 0.0, stream=s)
+    plan.exec()
+    plan.reset_data(dA2, dB2)
+    plan.exec()
+    plan.sync()
+    for A, dB in ((A1, dB1), (A2, dB2)):
+        exp = np.zeros(n)
+        oracle.transpose(A, exp, size, perm, 2.0, 0.0)
+        assert oracle.first_mismatch(host(dB), exp) == -1
+    text = plan.describe()
+    assert "path: tile" in text and "grid:" in text
+    # autotuned plan: same results, never touches the caller's B while tuning
+    dB3 = torch.full((n,), 7.0, dtype=torch.float64, device="cuda")
+    tuned = hptt_amd.create_plan_checked(dA1, dB3, size, perm, 2.0, 1.0, tuning_timeout=2.0)
+    torch.cuda.synchronize()
+    assert torch.all(dB3 == 7.0).item()
+    tuned.exec()
+    tuned.sync()
+    exp = np.full(n, 7.0)
+    oracle.transpose(A1, exp, size, perm, 2.0, 1.0)
+    assert oracle.first_mismatch(host(dB3), exp) == -1

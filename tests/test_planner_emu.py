@@ -1,0 +1,130 @@
+"""The planner and the kernels' index math, checked on the CPU.
+
+tests/emu/libemu.so runs the library's own block bodies (hptt_amd/csrc/
+hpttx_core.h) with the library's own planner on the host: every global and LDS
+access is bounds-checked, every store into B is counted (each element of the
+tensor exactly once, padding never), and the result is compared bit for bit
+with the oracle.  This is how launch shapes are shown to stay in bounds before
+they run on a GPU.
+"""
+import numpy as np
+import pytest
+
+import emu_runner as E
+import oracle
+import shapes
+
+
+def test_fastdiv_exact():
+    lib = E.lib()
+    for d in [1, 2, 3, 5, 7, 24, 32, 96, 384, 1000, 65537, 2**31 - 1, 2**31, 2**31 + 1,
+              2**32 - 1]:
+        assert lib.emu_check_fastdiv(d, 0, 200000) == -1, d
+        assert lib.emu_check_fastdiv(d, 2**32 - 200000, 2**32 - 1) == -1, d
+    rng = np.random.default_rng(0)
+    for d in rng.integers(1, 2**32 - 1, size=40):
+        lo = int(rng.integers(0, 2**32 - 20000))
+        assert lib.emu_check_fastdiv(int(d), lo, lo + 20000) == -1
+
+
+def _check(dt, size, perm, alpha, beta, io=None, oo=None, tune=None, cc=False,
+           offset=0, force_scalar=False):
+    n_a = int(np.prod(io if io else size))
+    out_size = [size[p] for p in perm]
+    n_b = int(np.prod(oo if oo else out_size))
+    Abuf = oracle.uniform(n_a + offset, dt, 101)
+    Bbuf = oracle.uniform(n_b + offset, dt, 202)
+    A = Abuf[offset:]
+    B0 = Bbuf[offset:]
+    exp = B0.copy()
+    assert oracle.transpose(A, exp, size, perm, alpha, beta, io, oo, complex_coef=cc) == 0
+    got = B0.copy() if offset == 0 else Bbuf.copy()[offset:]
+    rc, err, desc = E.transpose(A, got, size, perm, alpha, beta, io, oo, tune=tune,
+                                complex_coef=cc, force_scalar=force_scalar)
+    assert rc == 0, (size, perm, err, desc)
+    assert oracle.first_mismatch(got, exp) == -1, (size, perm, desc)
+    return desc
+
+
+@pytest.mark.parametrize("dt", shapes.DTYPES)
+def test_emulated_random_shapes(dt):
+    rng = np.random.default_rng(1234 + shapes.DTYPES.index(dt))
+    paths = set()
+    for k in range(60):
+        size, perm = shapes.random_case(rng, max_elems=6000)
+        al, be = [(1.0, 0.0), (2.3, 4.2), (-0.5, 1e-17), (1.25, -0.75)][k % 4]
+        desc = _check(dt, size, perm, al, be)
+        paths.add(desc.split("path: ")[1].split("\n")[0])
+    assert {"tile", "tile-shared", "axpby"} <= paths, paths
+
+
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
+def test_emulated_complex_coefficients(dt):
+    rng = np.random.default_rng(77)
+    for k in range(20):
+        size, perm = shapes.random_case(rng, max_elems=4000)
+        be = complex(0.7, -0.2) if k % 2 else 0j
+        _check(dt, size, perm, complex(1.1, 0.3), be, cc=True)
+
+
+@pytest.mark.parametrize("dt", shapes.DTYPES)
+def test_emulated_padded_and_misaligned(dt):
+    rng = np.random.default_rng(4321)
+    for k in range(25):
+        size, perm = shapes.random_case(rng, max_elems=3000, max_dim=16)
+        io, oo = shapes.random_outer(rng, size, perm)
+        _check(dt, size, perm, 2.3, 4.2 if k % 2 else 0.0, io, oo)
+    # pointers one element off a 16-byte boundary force the scalar kernels
+    for k in range(10):
+        size, perm = shapes.random_case(rng, max_elems=3000)
+        _check(dt, size, perm, 1.5, 0.25, offset=1)
+
+
+@pytest.mark.parametrize("tune", [[8, 8, 256, 64], [16, 64, 1024, 128], [128, 16, 4096, 1024],
+                                  [256, 256, 8192, 4096], [4, 4, 64, 16]])
+def test_emulated_tile_shapes(tune):
+    rng = np.random.default_rng(sum(tune))
+    for dt in shapes.DTYPES:
+        for k in range(8):
+            size, perm = shapes.random_case(rng, max_elems=5000)
+            _check(dt, size, perm, 2.3, 4.2 if k % 2 else 0.0, tune=tune)
+
+
+def test_emulated_benchmark_shapes_reduced():
+    # the benchmark workloads' perms on reduced extents (same fusion classes)
+    _check(np.float32, [16, 12, 8, 4, 6, 8], shapes.CONFIG2[1], 2.3, 4.2)
+    _check(np.float64, [16, 12, 8, 4, 6, 8], shapes.CONFIG2[1], 2.3, 4.2)
+    _check(np.complex128, [12, 10, 9, 8], [3, 2, 1, 0], 1.1, 0.7)
+    _check(np.complex128, [12, 10, 9, 8], [0, 3, 2, 1], 1.1, 0.7)
+    _check(np.float32, [16, 32, 64], [2, 1, 0], 1.0, 0.0)
+
+
+def test_plans_for_benchmark_shapes():
+    rc, err, d = E.plan(np.float32, *shapes.CONFIG2)
+    assert rc == 0 and "path: tile\n" in d and "vec(r,w)=4,4" in d, d
+    rc, err, d = E.plan(np.float64, *shapes.CONFIG2)
+    assert rc == 0 and "vec(r,w)=2,2" in d, d
+    rc, err, d = E.plan(np.complex128, *shapes.CONFIG3_TILE)
+    assert rc == 0 and "path: tile\n" in d, d
+    rc, err, d = E.plan(np.complex128, *shapes.CONFIG3_WIDE)
+    assert rc == 0 and "path: wide-copy" in d, d
+    rc, err, d = E.plan(np.float32, *shapes.CONFIG1)
+    assert rc == 0 and "path: tile\n" in d, d
+
+
+def test_fusion_and_validation():
+    # adjacent ascending pairs fuse (reference parameter_trans.tcc:235-250)
+    rc, err, d = E.plan(np.float32, [4, 5, 6, 7], [2, 3, 0, 1])
+    assert rc == 0 and "fused dims (A order, size:strideA:strideB): 20:1:42 42:20:1" in d, d
+    # identity fuses to one dim -> axpby (the reference crashes here)
+    rc, err, d = E.plan(np.float32, [4, 5, 6], [0, 1, 2])
+    assert rc == 0 and "path: axpby" in d
+    # padding blocks fusion
+    rc, err, d = E.plan(np.float32, [4, 5, 6, 7], [2, 3, 0, 1], [5, 5, 6, 7])
+    assert rc == 0 and d.count(":") >= 12
+    # invalid arguments
+    assert E.plan(np.float32, [4], [0])[0] == 1
+    assert E.plan(np.float32, [4, 4], [0, 0])[0] == 1
+    assert E.plan(np.float32, [4, 0], [1, 0])[0] == 1
+    assert E.plan(np.float32, [2] * 7, list(range(7)))[0] == 4
+    assert E.plan(np.float32, [4, 4], [1, 0], [3, 4])[0] == 1
