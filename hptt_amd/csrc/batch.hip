@@ -1,0 +1,410 @@
+// batch.hip -- persistent work-queue kernel over a batch of transpositions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <sstream>
+
+#include "batch.h"
+#include "hpttx_core.h"
+#include "tune.h"
+
+namespace hpttx {
+
+namespace {
+
+constexpr uint32_t kVarRows = 1, kVarVr = 2, kVarVw = 4, kVarUpd = 8, kVarSh = 16, kVarCc = 32;
+
+template <typename T, typename S, int NC, int VR, int VW, bool UPD, bool CC, bool SH>
+__device__ __forceinline__ void tile_one(const TileDesc &d, uint32_t lt, uint32_t tid,
+                                         const T *A, T *B, unsigned char *smem,
+                                         const Coef<S> &co) {
+  T *lds = reinterpret_cast<T *>(smem);
+  const TileTabs tb = tile_tabs(d, smem, sizeof(T));
+  const TileCoord tc = tile_coord(d, lt);
+  tile_tables(d, tc, tid, tb);
+  __syncthreads();
+  tile_read<T, VR>(d, tc, tid, A, lds, tb);
+  __syncthreads();
+  tile_write<T, S, NC, VW, UPD, CC, SH, !UPD>(d, tc, tid, B, lds, tb, co);
+  __syncthreads();
+}
+
+template <typename T, typename S, int NC, int V, bool UPD, bool CC>
+__device__ __forceinline__ void rows_one(const RowDesc &d, uint32_t lt, uint32_t tid,
+                                         const T *A, T *B, unsigned char *smem,
+                                         const Coef<S> &co) {
+  int64_t *tabA = reinterpret_cast<int64_t *>(smem);
+  int64_t *tabB = tabA + d.RB;
+  const RowCoord rc = row_coord(d, lt);
+  row_tables(d, rc, tid, tabA, tabB);
+  __syncthreads();
+  row_body<T, S, NC, V, UPD, CC, !UPD>(d, rc, tid, A, B, tabA, tabB, co);
+  __syncthreads();
+}
+
+// Variant codes that exist for an element type: rows ignore the write-vector
+// and shared bits; scalar-only types (VCAP == 1) ignore the vector bits;
+// complex coefficients only for complex types.
+template <int VCAP, bool CCOK>
+constexpr bool valid_code(int c) {
+  return !((c & kVarRows) && (c & (kVarVw | kVarSh))) &&
+         !(VCAP == 1 && (c & (kVarVr | kVarVw))) && !(!CCOK && (c & kVarCc));
+}
+
+template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
+__device__ __forceinline__ void run_code(const BatchEntry &e, uint32_t lt, uint32_t tid,
+                                         unsigned char *smem) {
+  constexpr bool rows = CODE & kVarRows;
+  constexpr int vr = (CODE & kVarVr) ? VCAP : 1;
+  constexpr int vw = (CODE & kVarVw) ? VCAP : 1;
+  constexpr bool upd = CODE & kVarUpd;
+  constexpr bool sh = CODE & kVarSh;
+  constexpr bool cc = CODE & kVarCc;
+  const Coef<S> co{(S)e.ar, (S)e.ai, (S)e.br, (S)e.bi};
+  const T *A = static_cast<const T *>(e.A);
+  T *B = static_cast<T *>(e.B);
+  if constexpr (rows)
+    rows_one<T, S, NC, vr, upd, cc>(e.u.rows, lt, tid, A, B, smem, co);
+  else
+    tile_one<T, S, NC, vr, vw, upd, cc, sh>(e.u.tile, lt, tid, A, B, smem, co);
+}
+
+template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
+__device__ __forceinline__ void dispatch(uint32_t v, const BatchEntry &e, uint32_t lt,
+                                         uint32_t tid, unsigned char *smem) {
+  if constexpr (CODE < 64) {
+    if constexpr (valid_code<VCAP, CCOK>(CODE)) {
+      if (v == CODE) {
+        run_code<T, S, NC, VCAP, CCOK, CODE>(e, lt, tid, smem);
+        return;
+      }
+    }
+    dispatch<T, S, NC, VCAP, CCOK, CODE + 1>(v, e, lt, tid, smem);
+  }
+}
+
+template <typename T, typename S, int NC, int VCAP, bool CCOK>
+__global__ __launch_bounds__(kBlock) void k_batch(const BatchEntry *__restrict__ ents,
+                                                  const uint32_t *__restrict__ t2e,
+                                                  uint32_t total, uint32_t *counter) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint32_t s_range[2];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t G = gridDim.x;
+  for (;;) {
+    if (tid == 0) {
+      // Guided self-scheduling: chunks shrink as the queue drains.
+      const uint32_t cur = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t rem = cur < total ? total - cur : 0;
+      uint32_t chunk = rem / (4 * G);
+      chunk = chunk < 1 ? 1 : (chunk > 16 ? 16 : chunk);
+      const uint32_t old = atomicAdd(counter, chunk);
+      s_range[0] = old;
+      s_range[1] = old < total ? umin32(old + chunk, total) : old;
+    }
+    __syncthreads();
+    const uint32_t t0 = __builtin_amdgcn_readfirstlane(s_range[0]);
+    const uint32_t t1 = __builtin_amdgcn_readfirstlane(s_range[1]);
+    __syncthreads();
+    if (t0 >= total) break;
+    for (uint32_t t = t0; t < t1; ++t) {
+      const uint32_t ei = __builtin_amdgcn_readfirstlane(t2e[t]);
+      const BatchEntry &e = ents[ei];
+      const uint32_t v = __builtin_amdgcn_readfirstlane(e.variant);
+      const uint32_t lt = t - e.tile_begin;
+      dispatch<T, S, NC, VCAP, CCOK, 0>(v, e, lt, tid, smem);
+    }
+  }
+}
+
+__global__ void k_reset(uint32_t *c, uint32_t n) {
+  if (threadIdx.x < n) c[threadIdx.x] = 0;
+}
+
+const void *batch_kernel(int dtype) {
+  switch (dtype) {
+    case kF32: return reinterpret_cast<const void *>(&k_batch<Elem<float, 1>, float, 1, 4, false>);
+    case kF64: return reinterpret_cast<const void *>(&k_batch<Elem<double, 1>, double, 1, 2, false>);
+    case kC64: return reinterpret_cast<const void *>(&k_batch<Elem<float, 2>, float, 2, 2, true>);
+    case kC128: return reinterpret_cast<const void *>(&k_batch<Elem<double, 2>, double, 2, 1, true>);
+  }
+  return nullptr;
+}
+
+hipError_t launch_group(const BatchGroup &g, uint32_t *counter, hipStream_t st) {
+  dim3 grid(g.grid), block(kBlock);
+  switch (g.dtype) {
+    case kF32:
+      hipLaunchKernelGGL((k_batch<Elem<float, 1>, float, 1, 4, false>), grid, block, g.lds_bytes,
+                         st, g.d_ent, g.d_tile2ent, g.total_tiles, counter);
+      break;
+    case kF64:
+      hipLaunchKernelGGL((k_batch<Elem<double, 1>, double, 1, 2, false>), grid, block,
+                         g.lds_bytes, st, g.d_ent, g.d_tile2ent, g.total_tiles, counter);
+      break;
+    case kC64:
+      hipLaunchKernelGGL((k_batch<Elem<float, 2>, float, 2, 2, true>), grid, block, g.lds_bytes,
+                         st, g.d_ent, g.d_tile2ent, g.total_tiles, counter);
+      break;
+    case kC128:
+      hipLaunchKernelGGL((k_batch<Elem<double, 2>, double, 2, 1, true>), grid, block,
+                         g.lds_bytes, st, g.d_ent, g.d_tile2ent, g.total_tiles, counter);
+      break;
+  }
+  return hipGetLastError();
+}
+
+// Tile-shape class of a problem: the autotuner's key for batches.
+std::string class_key(const Problem &p, const LaunchPlan &lp, bool update) {
+  std::ostringstream os;
+  int bucket = 0;
+  for (uint64_t n = p.N; n > 1; n >>= 2) ++bucket;
+  os << p.dtype << '|' << p.fr << '|' << lp.path << '|' << bucket << '|' << update;
+  return os.str();
+}
+
+uint32_t entry_variant(const LaunchPlan &lp, bool update, bool cc, uint32_t vcap) {
+  uint32_t v = 0;
+  if (lp.kind == kRows) {
+    v |= kVarRows;
+    if (lp.rows.v == vcap && vcap > 1) v |= kVarVr;
+  } else {
+    if (lp.tile.vr == vcap && vcap > 1) v |= kVarVr;
+    if (lp.tile.vw == vcap && vcap > 1) v |= kVarVw;
+    if (lp.tile.VS > 1) v |= kVarSh;
+  }
+  if (update) v |= kVarUpd;
+  if (cc) v |= kVarCc;
+  return v;
+}
+
+// Finalize vectors for the pointers (full 16-byte vectors or scalars).
+void pick_vectors(const Problem &p, LaunchPlan &lp, const void *A, const void *B) {
+  const uint32_t cap = 16 / p.esize;
+  uint32_t vr = lp.vr_shape, vw = lp.vw_shape;
+  if (lp.kind == kRows) {
+    uint32_t v = vr;
+    if (v > align_vec(A, p.esize, cap) || v > align_vec(B, p.esize, cap) || v != cap) v = 1;
+    vr = vw = v;
+  } else {
+    if (vr > align_vec(A, p.esize, cap) || vr != cap) vr = 1;
+    if (vw > align_vec(B, p.esize, cap) || vw != cap) vw = 1;
+  }
+  finalize_vectors(p, lp, vr, vw);
+}
+
+}  // namespace
+
+int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, double tuning_timeout,
+                Batch &b, std::string &err) {
+  b = Batch();
+  b.count = (uint32_t)items.size();
+  b.stream = stream;
+  b.paths.resize(items.size());
+  struct Planned {
+    Problem p;
+    LaunchPlan lp;
+    bool update;
+    std::string key;
+  };
+  std::vector<Planned> pl(items.size());
+  std::map<std::string, TuneParams> class_tune;
+  std::map<std::string, size_t> class_rep;   // largest member of each class
+  for (size_t k = 0; k < items.size(); ++k) {
+    const BatchItemSpec &it = items[k];
+    if (!it.A || !it.B) { err = "item " + std::to_string(k) + ": null data pointer"; return kEInval; }
+    bool padded_in = false, padded_out = false;
+    for (uint32_t d = 0; d < it.rank && d < (uint32_t)kMaxRank; ++d) {
+      padded_in |= it.in_outer[d] != 0;
+      padded_out |= it.out_outer[d] != 0;
+    }
+    int rc = build_problem(it.dtype, it.rank, it.in_size, it.perm, padded_in ? it.in_outer : nullptr,
+                           padded_out ? it.out_outer : nullptr, pl[k].p, err);
+    if (rc != kOk) { err = "item " + std::to_string(k) + ": " + err; return rc; }
+    const bool cc = it.cc && (it.coef.ai != 0.0 || it.coef.bi != 0.0);
+    pl[k].update = cc ? (std::fabs(it.coef.br) >= 1e-16 || std::fabs(it.coef.bi) >= 1e-16)
+                      : std::fabs(it.coef.br) >= 1e-16;
+    rc = plan_launch(pl[k].p, TuneParams{}, pl[k].lp, err);
+    if (rc != kOk) { err = "item " + std::to_string(k) + ": " + err; return rc; }
+    pl[k].key = class_key(pl[k].p, pl[k].lp, pl[k].update);
+    auto it2 = class_rep.find(pl[k].key);
+    if (it2 == class_rep.end() || pl[it2->second].p.N < pl[k].p.N) class_rep[pl[k].key] = k;
+  }
+  // Per-class tile shapes: tuned on the class's largest member (scratch
+  // buffers), or taken from the tuning cache.
+  if (tuning_timeout != 0.0) {
+    const double share = tuning_timeout > 0 ? tuning_timeout / std::max<size_t>(1, class_rep.size()) : -1.0;
+    for (auto &kv : class_rep) {
+      const Planned &rep = pl[kv.second];
+      const BatchItemSpec &it = items[kv.second];
+      const bool cc = it.cc && (it.coef.ai != 0.0 || it.coef.bi != 0.0);
+      TuneParams tp{};
+      if (!tune_lookup(rep.p, rep.update, tp)) {
+        int rc = autotune(rep.p, it.coef, rep.update, cc, share, stream, tp, err);
+        if (rc != kOk) return rc;
+      }
+      class_tune[kv.first] = tp;
+    }
+  }
+  std::map<int, std::vector<size_t>> by_dtype;
+  for (size_t k = 0; k < items.size(); ++k) {
+    auto ct = class_tune.find(pl[k].key);
+    if (ct != class_tune.end()) {
+      int rc = plan_launch(pl[k].p, ct->second, pl[k].lp, err);
+      if (rc != kOk) return rc;
+    }
+    pick_vectors(pl[k].p, pl[k].lp, items[k].A, items[k].B);
+    b.paths[k] = pl[k].lp.path;
+    by_dtype[items[k].dtype].push_back(k);
+  }
+  hipError_t he = hipSuccess;
+  for (auto &kv : by_dtype) {
+    BatchGroup g;
+    g.dtype = kv.first;
+    std::vector<size_t> idx = kv.second;
+    // Largest transposition first: its tiles are handed out first and the
+    // small ones fill the tail.
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) {
+      return pl[x].p.N > pl[y].p.N;
+    });
+    const uint32_t vcap = 16 / dtype_esize(g.dtype);
+    uint64_t tiles = 0;
+    for (size_t k : idx) {
+      const Planned &q = pl[k];
+      const BatchItemSpec &it = items[k];
+      const bool cc = it.cc && (it.coef.ai != 0.0 || it.coef.bi != 0.0);
+      BatchEntry e;
+      std::memset(&e, 0, sizeof(e));
+      e.variant = entry_variant(q.lp, q.update, cc, vcap);
+      e.tile_begin = (uint32_t)tiles;
+      e.ntiles = q.lp.kind == kTile ? q.lp.tile.ntiles : q.lp.rows.ntiles;
+      e.lds_bytes = (uint32_t)q.lp.lds_bytes;
+      e.A = it.A;
+      e.B = it.B;
+      e.ar = it.coef.ar;
+      e.ai = cc ? it.coef.ai : 0.0;
+      e.br = it.coef.br;
+      e.bi = cc ? it.coef.bi : 0.0;
+      if (q.lp.kind == kTile) e.u.tile = q.lp.tile;
+      else e.u.rows = q.lp.rows;
+      tiles += e.ntiles;
+      g.lds_bytes = std::max(g.lds_bytes, e.lds_bytes);
+      g.bytes += q.p.N * q.p.esize * (uint64_t)(q.update ? 3 : 2);
+      g.h_ent.push_back(e);
+      g.order.push_back((uint32_t)k);
+    }
+    if (tiles > 0xFFFFFFFFull) { err = "batch has more than 2^32 tiles"; return kEUnsupported; }
+    g.nent = (uint32_t)g.h_ent.size();
+    g.total_tiles = (uint32_t)tiles;
+    std::vector<uint32_t> t2e(tiles);
+    for (uint32_t k = 0; k < g.nent; ++k)
+      std::fill(t2e.begin() + g.h_ent[k].tile_begin,
+                t2e.begin() + g.h_ent[k].tile_begin + g.h_ent[k].ntiles, k);
+    int dev = 0, ncu = 256, occ = 1;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_kernel(g.dtype), kBlock,
+                                                     g.lds_bytes) != hipSuccess || occ < 1)
+      occ = 1;
+    g.grid = (uint32_t)std::min<uint64_t>(std::max<uint32_t>(1, g.total_tiles), (uint64_t)ncu * occ);
+    he = hipMalloc(&g.d_ent, sizeof(BatchEntry) * g.nent);
+    if (he == hipSuccess) he = hipMalloc(&g.d_tile2ent, sizeof(uint32_t) * std::max<uint64_t>(1, tiles));
+    if (he == hipSuccess)
+      he = hipMemcpy(g.d_ent, g.h_ent.data(), sizeof(BatchEntry) * g.nent, hipMemcpyHostToDevice);
+    if (he == hipSuccess && tiles)
+      he = hipMemcpy(g.d_tile2ent, t2e.data(), sizeof(uint32_t) * tiles, hipMemcpyHostToDevice);
+    b.bytes += g.bytes;
+    b.groups.push_back(std::move(g));
+    if (he != hipSuccess) break;
+  }
+  if (he == hipSuccess) he = hipMalloc(&b.d_counters, sizeof(uint32_t) * std::max<size_t>(1, b.groups.size()));
+  if (he == hipSuccess) he = hipMemset(b.d_counters, 0, sizeof(uint32_t) * std::max<size_t>(1, b.groups.size()));
+  if (he != hipSuccess) {
+    err = std::string("batch setup: ") + hipGetErrorString(he);
+    batch_free(b);
+    return he == hipErrorOutOfMemory ? kENoMem : kEHip;
+  }
+  return kOk;
+}
+
+hipError_t batch_enqueue(Batch &b, hipStream_t st) {
+  if (b.groups.empty()) return hipSuccess;
+  // Counters are cleared by a kernel (not a memset node) so the sequence can
+  // be captured in a graph and replayed.
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, b.d_counters, (uint32_t)b.groups.size());
+  hipError_t e = hipGetLastError();
+  for (size_t i = 0; i < b.groups.size() && e == hipSuccess; ++i)
+    if (b.groups[i].total_tiles) e = launch_group(b.groups[i], b.d_counters + i, st);
+  return e;
+}
+
+hipError_t batch_capture(Batch &b) {
+  hipStream_t cs = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e != hipSuccess) return e;
+  e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    hipError_t e2 = batch_enqueue(b, cs);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(cs, &g);
+    if (e2 != hipSuccess) e = e2;
+    if (e == hipSuccess) {
+      b.graph = g;
+      e = hipGraphInstantiate(&b.gexec, g, nullptr, nullptr, 0);
+    } else if (g) {
+      (void)hipGraphDestroy(g);
+    }
+  }
+  (void)hipStreamDestroy(cs);
+  return e;
+}
+
+void batch_free(Batch &b) {
+  if (b.gexec) (void)hipGraphExecDestroy(b.gexec);
+  if (b.graph) (void)hipGraphDestroy(b.graph);
+  for (BatchGroup &g : b.groups) {
+    if (g.d_ent) (void)hipFree(g.d_ent);
+    if (g.d_tile2ent) (void)hipFree(g.d_tile2ent);
+    g.d_ent = nullptr;
+    g.d_tile2ent = nullptr;
+  }
+  if (b.d_counters) (void)hipFree(b.d_counters);
+  b.d_counters = nullptr;
+  b.gexec = nullptr;
+  b.graph = nullptr;
+  b.groups.clear();
+}
+
+int batch_set_data(Batch &b, uint32_t k, const void *A, void *B, std::string &err) {
+  if (!A || !B) { err = "null data pointer"; return kEInval; }
+  for (BatchGroup &g : b.groups) {
+    for (uint32_t i = 0; i < g.nent; ++i) {
+      if (g.order[i] != k) continue;
+      BatchEntry &e = g.h_ent[i];
+      const uint32_t es = dtype_esize(g.dtype);
+      // The entry's vector widths were chosen for the old pointers' 16-byte
+      // alignment; new pointers must allow the same widths.
+      const bool vr = (e.variant & kVarVr) != 0, vw = (e.variant & kVarVw) != 0;
+      const bool rows = (e.variant & kVarRows) != 0;
+      const uint32_t cap = 16 / es;
+      if ((vr && align_vec(A, es, cap) != cap) || ((vw || (rows && vr)) && align_vec(B, es, cap) != cap)) {
+        err = "new pointers are not 16-byte aligned like the ones the batch was built for";
+        return kEInval;
+      }
+      e.A = A;
+      e.B = B;
+      hipError_t he = hipMemcpyAsync(&g.d_ent[i], &e, sizeof(BatchEntry), hipMemcpyHostToDevice, b.stream);
+      if (he == hipSuccess) he = hipStreamSynchronize(b.stream);
+      if (he != hipSuccess) { err = hipGetErrorString(he); return kEHip; }
+      return kOk;
+    }
+  }
+  err = "item index out of range";
+  return kEInval;
+}
+
+}  // namespace hpttx

@@ -103,6 +103,10 @@ int create_common(int dtype, const void *in, void *out, uint32_t rank,
 
 }  // namespace
 
+namespace hpttx {
+void set_last_error(const std::string &m) { g_err = m; }
+}  // namespace hpttx
+
 extern "C" {
 
 int hpttx_plan_create_s(const float *in, float *out, uint32_t rank, const uint32_t *in_size,

@@ -148,6 +148,62 @@ int hpttx_plan_get_info(const hpttx_plan *plan, hpttx_plan_info *info);
 /* Release the plan (the reference's shared_ptr deleter, src/hptt.cc:89-92). */
 void hpttx_plan_destroy(hpttx_plan *plan);
 
+/*
+ * Batched transpositions: many independent plans in one persistent,
+ * work-queue kernel launch per element type, captured once in a hipGraph and
+ * replayed by hpttx_batch_exec.  No counterpart in the reference, whose only
+ * way to run several transpositions is one plan->exec() after another
+ * (src/hptt.cc:16-93, inc/hptt/cgraph/cgraph_trans.tcc:27-48); each item has
+ * the meaning of one hpttx_plan_create_* call.  Tile shapes are chosen per
+ * (dtype, fused rank, path, size class); with tuning_timeout != 0 each class
+ * is autotuned on scratch buffers.
+ */
+typedef struct hpttx_batch hpttx_batch;
+
+typedef struct hpttx_batch_item {
+  int32_t dtype;               /* HPTTX_F32 / F64 / C64 / C128 */
+  uint32_t rank;               /* 2..6 */
+  const void *in;              /* device A */
+  void *out;                   /* device B */
+  uint32_t in_size[6];
+  uint32_t perm[6];
+  uint32_t in_outer[6];        /* all zero = unpadded */
+  uint32_t out_outer[6];       /* all zero = unpadded */
+  double alpha_re, alpha_im;   /* real dtypes use the _re parts */
+  double beta_re, beta_im;
+  int32_t complex_coef;        /* nonzero: complex alpha/beta (C64/C128) */
+  int32_t reserved_;
+} hpttx_batch_item;
+
+int hpttx_batch_create(uint32_t count, const hpttx_batch_item *items,
+                       double tuning_timeout, void *stream,
+                       hpttx_batch **batch);
+/* Enqueue every item of the batch (graph replay) on the batch's stream. */
+int hpttx_batch_exec(hpttx_batch *batch);
+int hpttx_batch_exec_on(hpttx_batch *batch, void *stream);
+int hpttx_batch_sync(hpttx_batch *batch);
+/* Re-point item `index` (creation order) at new buffers with the same
+ * 16-byte alignment; takes effect at the next exec. */
+int hpttx_batch_set_data(hpttx_batch *batch, uint32_t index, const void *in,
+                         void *out);
+/* 1 = replay a captured hipGraph (default), 0 = launch directly. */
+int hpttx_batch_use_graph(hpttx_batch *batch, int enable);
+
+typedef struct hpttx_batch_info {
+  uint32_t count;              /* items */
+  uint32_t groups;             /* element types present = launches per exec */
+  uint64_t algorithmic_bytes;  /* sum over items of N*elem*(2 + update) */
+  uint64_t tiles;              /* block tiles over all groups */
+  uint32_t grid[4];            /* persistent blocks per group */
+  uint32_t lds_bytes[4];
+} hpttx_batch_info;
+int hpttx_batch_get_info(const hpttx_batch *batch, hpttx_batch_info *info);
+/* Path of item `index` ("tile", "tile-shared", "wide-copy", "axpby",
+ * "gather") into buf. */
+int hpttx_batch_item_path(const hpttx_batch *batch, uint32_t index, char *buf,
+                          size_t len);
+void hpttx_batch_destroy(hpttx_batch *batch);
+
 /* Reason for the last failure on this thread ("" if none). */
 const char *hpttx_last_error(void);
 int hpttx_version(void);
