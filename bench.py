@@ -39,6 +39,10 @@ WORKLOADS = {
                          alpha=complex(1.1, 0.3), beta=complex(0.7, -0.2), flush=False),
     "3d-f32-copy": dict(size=[64, 128, 256], perm=[2, 1, 0], dtype="float32",
                         alpha=1.0, beta=0.0, flush=True),
+    # identity perm (axpby path): a plain streaming copy of 2 x 512 MiB, the
+    # practical ceiling the transposes are compared with
+    "copy-f32-512m": dict(size=[131072, 1024], perm=[0, 1], dtype="float32",
+                          alpha=1.0, beta=0.0, flush=False),
 }
 
 
@@ -46,8 +50,9 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def time_workload(name, steps, warmup):
-    """Returns a dict with GB/s, ms per exec and plan info for one workload."""
+def time_workload(name, steps, warmup, tune=0.0):
+    """Returns a dict with GB/s, ms per exec and plan info for one workload
+    (tune > 0: autotune the plan for that many seconds first)."""
     import torch
 
     import hptt_amd
@@ -60,8 +65,9 @@ def time_workload(name, steps, warmup):
     stream = torch.cuda.Stream()
     torch.cuda.synchronize()
     plan = hptt_amd.create_plan_checked(A, B, w["size"], w["perm"], w["alpha"], w["beta"],
-                                        stream=stream)
+                                        stream=stream, tuning_timeout=tune)
     info = plan.info()
+    log(plan.describe().rstrip())
     nbytes = info.algorithmic_bytes
     flush = torch.empty(512 * 2**20 // 4, device="cuda", dtype=torch.float32) if w["flush"] else None
     for _ in range(warmup):
@@ -111,6 +117,95 @@ def time_workload(name, steps, warmup):
     return res
 
 
+def time_batched(dtype_name, reps=5, seq_reps=3):
+    """The batched workload: 4096 tensors (seeded manifest, rank 2..6, dims
+    log-uniform in [2,128], <= 4 Mi elements each, random perms, 25% beta=0)
+    as one Batch (persistent kernel, graph replay) against one plan->exec()
+    per tensor in sequence on the same stream; identical bits required."""
+    import torch
+
+    import hptt_amd
+    from hptt_amd.workloads import batch_manifest, layout
+    man = batch_manifest()
+    tdt = getattr(torch, dtype_name)
+    esz = torch.empty(0, dtype=tdt).element_size()
+    offs, total = layout(man, esz)
+    real = torch.float32 if dtype_name in ("float32", "complex64") else torch.float64
+    ncomp = 2 if "complex" in dtype_name else 1
+    g = torch.Generator(device="cuda").manual_seed(4096)
+    A = torch.empty(total * ncomp, dtype=real, device="cuda").uniform_(-1, 1, generator=g)
+    B0 = torch.empty(total * ncomp, dtype=real, device="cuda").uniform_(-1, 1, generator=g)
+    A = A.view(tdt) if ncomp == 2 else A
+    B0 = B0.view(tdt) if ncomp == 2 else B0
+    B = B0.clone()
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    items, plans = [], []
+    for m, o in zip(man, offs):
+        n = int(np.prod(m["in_size"]))
+        a, b = A[o:o + n], B[o:o + n]
+        items.append(hptt_amd.make_batch_item(a, b, m["in_size"], m["perm"], m["alpha"], m["beta"]))
+        plans.append(hptt_amd.create_plan_checked(a, b, m["in_size"], m["perm"], m["alpha"],
+                                                  m["beta"], stream=stream))
+    t_build = time.perf_counter()
+    batch = hptt_amd.Batch(items, stream=stream)
+    t_build = time.perf_counter() - t_build
+    info = batch.info()
+    exec_fn = hptt_amd.lib().hpttx_plan_exec
+    handles = [p._h for p in plans]
+
+    def run_seq():
+        for h in handles:
+            if exec_fn(h) != 0:
+                raise RuntimeError(hptt_amd.last_error())
+
+    # warm up both (the first batch exec captures the graph)
+    run_seq()
+    batch.exec()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    seq_ms = []
+    for _ in range(seq_reps):
+        e0.record(stream)
+        run_seq()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        seq_ms.append(e0.elapsed_time(e1))
+    bat_ms = []
+    for _ in range(reps):
+        e0.record(stream)
+        batch.exec()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        bat_ms.append(e0.elapsed_time(e1))
+    # identical output: sequential plans vs the batch from the same B0
+    B.copy_(B0)
+    torch.cuda.synchronize()
+    run_seq()
+    torch.cuda.synchronize()
+    Bseq = B.clone()
+    B.copy_(B0)
+    torch.cuda.synchronize()
+    batch.exec()
+    torch.cuda.synchronize()
+    iv = torch.int32 if esz == 4 else torch.int64
+    same = bool(torch.equal(B.view(real).view(iv), Bseq.view(real).view(iv)))
+    seq = float(np.median(seq_ms))
+    bat = float(np.median(bat_ms))
+    res = {"tensors": len(man), "batched_ms": round(bat, 3), "sequential_ms": round(seq, 3),
+           "batched_transposes_per_s": round(len(man) / (bat * 1e-3), 1),
+           "sequential_transposes_per_s": round(len(man) / (seq * 1e-3), 1),
+           "speedup": round(seq / bat, 2),
+           "batched_gbps": round(info.algorithmic_bytes / (bat * 1e-3) / 1e9, 1),
+           "sequential_gbps": round(info.algorithmic_bytes / (seq * 1e-3) / 1e9, 1),
+           "algorithmic_bytes": info.algorithmic_bytes, "tiles": info.tiles,
+           "grid": list(info.grid)[:info.groups], "build_s": round(t_build, 3),
+           "identical_to_sequential": same}
+    del batch, plans, A, B, B0, Bseq
+    torch.cuda.empty_cache()
+    return res
+
+
 def cpu_baseline(budget_s=12.0):
     """Oracle (port of the reference's OpenMP RefTrans) on the fp32 workload."""
     import oracle
@@ -142,6 +237,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="main workload only")
+    ap.add_argument("--tune", type=float, default=0.0,
+                    help="autotune each plan for this many seconds before timing")
+    ap.add_argument("--workload", default="6d-f32",
+                    help="main workload (%s) or batched-f32 / batched-f64" % ", ".join(WORKLOADS))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -153,17 +252,30 @@ def main():
         sys.exit(1)
     import hptt_amd
     hptt_amd.lib()
-    log("bench: main workload 6d-f32, %d steps" % args.steps)
+    if args.workload.startswith("batched-"):
+        res = time_batched({"batched-f32": "float32", "batched-f64": "float64"}[args.workload],
+                           reps=max(1, args.steps))
+        print(json.dumps({"workload": args.workload, **res}), flush=True)
+        return
+    log("bench: main workload %s, %d steps" % (args.workload, args.steps))
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    main_res = time_workload("6d-f32", args.steps, args.warmup)
+    main_res = time_workload(args.workload, args.steps, args.warmup, args.tune)
     torch.cuda.synchronize()
-    log("bench: 6d-f32 %.1f GB/s (%.1f%% of peak)" % (main_res["gbps"], main_res["pct_hbm_peak"]))
+    log("bench: %s %.1f GB/s (%.1f%% of peak)" % (args.workload, main_res["gbps"],
+                                                 main_res["pct_hbm_peak"]))
     secondary = {}
     if not args.quick:
-        for name in ("6d-f64", "4d-c128-tile", "4d-c128-wide", "3d-f32-copy"):
-            secondary[name] = time_workload(name, max(10, args.steps // 2), args.warmup)
+        for name in ("6d-f64", "4d-c128-tile", "4d-c128-wide", "3d-f32-copy", "copy-f32-512m"):
+            if name == args.workload:
+                continue
+            secondary[name] = time_workload(name, max(10, args.steps // 2), args.warmup, args.tune)
             log("bench: %s %.1f GB/s" % (name, secondary[name]["gbps"]))
+        for name in ("batched-f32", "batched-f64"):
+            secondary[name] = time_batched(name.split("-")[1].replace("f", "float"))
+            log("bench: %s %s transposes/s batched vs %s sequential (x%.2f), identical=%s"
+                % (name, secondary[name]["batched_transposes_per_s"],
+                   secondary[name]["sequential_transposes_per_s"], secondary[name]["speedup"],
+                   secondary[name]["identical_to_sequential"]))
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, OpenMP)")

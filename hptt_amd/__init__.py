@@ -90,6 +90,7 @@ def lib() -> ctypes.CDLL:
     L.hpttx_last_error.argtypes = []
     L.hpttx_version.restype = ctypes.c_int
     L.hpttx_version.argtypes = []
+    _bind_batch(L)
     _lib = L
     return L
 
@@ -252,6 +253,126 @@ def create_plan(in_data, out_data, in_size, perm, alpha, beta,
                                    out_outer_size, stream, dtype)
     except HpttError:
         return None
+
+
+class BatchItem(ctypes.Structure):
+    """include/hptt_amd.h hpttx_batch_item."""
+    _fields_ = [
+        ("dtype", ctypes.c_int32), ("rank", ctypes.c_uint32),
+        ("in_", ctypes.c_void_p), ("out", ctypes.c_void_p),
+        ("in_size", ctypes.c_uint32 * 6), ("perm", ctypes.c_uint32 * 6),
+        ("in_outer", ctypes.c_uint32 * 6), ("out_outer", ctypes.c_uint32 * 6),
+        ("alpha_re", ctypes.c_double), ("alpha_im", ctypes.c_double),
+        ("beta_re", ctypes.c_double), ("beta_im", ctypes.c_double),
+        ("complex_coef", ctypes.c_int32), ("reserved_", ctypes.c_int32)]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint32), ("groups", ctypes.c_uint32),
+                ("algorithmic_bytes", ctypes.c_uint64), ("tiles", ctypes.c_uint64),
+                ("grid", ctypes.c_uint32 * 4), ("lds_bytes", ctypes.c_uint32 * 4)]
+
+
+def _bind_batch(L):
+    vp, u32, d = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_double
+    L.hpttx_batch_create.restype = ctypes.c_int
+    L.hpttx_batch_create.argtypes = [u32, ctypes.POINTER(BatchItem), d, vp,
+                                     ctypes.POINTER(ctypes.c_void_p)]
+    for name in ("hpttx_batch_exec", "hpttx_batch_sync"):
+        getattr(L, name).restype = ctypes.c_int
+        getattr(L, name).argtypes = [vp]
+    L.hpttx_batch_exec_on.restype = ctypes.c_int
+    L.hpttx_batch_exec_on.argtypes = [vp, vp]
+    L.hpttx_batch_set_data.restype = ctypes.c_int
+    L.hpttx_batch_set_data.argtypes = [vp, u32, vp, vp]
+    L.hpttx_batch_use_graph.restype = ctypes.c_int
+    L.hpttx_batch_use_graph.argtypes = [vp, ctypes.c_int]
+    L.hpttx_batch_get_info.restype = ctypes.c_int
+    L.hpttx_batch_get_info.argtypes = [vp, ctypes.POINTER(BatchInfo)]
+    L.hpttx_batch_item_path.restype = ctypes.c_int
+    L.hpttx_batch_item_path.argtypes = [vp, u32, ctypes.c_char_p, ctypes.c_size_t]
+    L.hpttx_batch_destroy.restype = None
+    L.hpttx_batch_destroy.argtypes = [vp]
+
+
+def make_batch_item(in_data, out_data, in_size, perm, alpha, beta, dtype=None,
+                    in_outer_size=(), out_outer_size=()) -> BatchItem:
+    """One hpttx_batch_item; in/out are device tensors or raw addresses."""
+    code = dtype_code(dtype) if dtype is not None else None
+    a, ca = _addr_and_dtype(in_data, code)
+    b, cb = _addr_and_dtype(out_data, code)
+    if ca != cb:
+        raise TypeError("A and B must have the same dtype")
+    it = BatchItem()
+    it.dtype = ca
+    it.rank = len(perm)
+    it.in_ = a
+    it.out = b
+    for k, v in enumerate(in_size):
+        it.in_size[k] = int(v)
+    for k, v in enumerate(perm):
+        it.perm[k] = int(v)
+    for k, v in enumerate(in_outer_size or ()):
+        it.in_outer[k] = int(v)
+    for k, v in enumerate(out_outer_size or ()):
+        it.out_outer[k] = int(v)
+    cc = isinstance(alpha, complex) or isinstance(beta, complex)
+    al, be = complex(alpha), complex(beta)
+    it.alpha_re, it.alpha_im = al.real, al.imag
+    it.beta_re, it.beta_im = be.real, be.imag
+    it.complex_coef = 1 if cc else 0
+    return it
+
+
+class Batch:
+    """Many transpositions executed by one persistent work-queue launch per
+    element type, replayed from a captured hipGraph (hpttx_batch_* in
+    include/hptt_amd.h).  items: a sequence of BatchItem."""
+
+    def __init__(self, items, tuning_timeout: float = 0.0, stream=None, keep=()):
+        L = lib()
+        arr = (BatchItem * len(items))(*items)
+        h = ctypes.c_void_p()
+        _check(L.hpttx_batch_create(len(items), arr, float(tuning_timeout),
+                                    _stream_ptr(stream), ctypes.byref(h)))
+        self._h = h
+        self.count = len(items)
+        self._keep = list(keep)
+
+    def exec(self) -> None:
+        _check(lib().hpttx_batch_exec(self._h))
+
+    __call__ = exec
+
+    def exec_on(self, stream) -> None:
+        _check(lib().hpttx_batch_exec_on(self._h, _stream_ptr(stream)))
+
+    def sync(self) -> None:
+        _check(lib().hpttx_batch_sync(self._h))
+
+    def use_graph(self, enable: bool) -> None:
+        _check(lib().hpttx_batch_use_graph(self._h, 1 if enable else 0))
+
+    def set_data(self, index: int, in_data, out_data, dtype=None) -> None:
+        a, _ = _addr_and_dtype(in_data, dtype)
+        b, _ = _addr_and_dtype(out_data, dtype)
+        _check(lib().hpttx_batch_set_data(self._h, index, a, b))
+
+    def info(self) -> BatchInfo:
+        inf = BatchInfo()
+        _check(lib().hpttx_batch_get_info(self._h, ctypes.byref(inf)))
+        return inf
+
+    def item_path(self, index: int) -> str:
+        buf = ctypes.create_string_buffer(32)
+        _check(lib().hpttx_batch_item_path(self._h, index, buf, 32))
+        return buf.value.decode()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.hpttx_batch_destroy(h)
+            self._h = ctypes.c_void_p(0)
 
 
 def algorithmic_bytes(n_elements: int, elem_bytes: int, update: bool) -> int:

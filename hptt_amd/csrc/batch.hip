@@ -27,9 +27,11 @@ __device__ __forceinline__ void tile_one(const TileDesc &d, uint32_t lt, uint32_
   const TileCoord tc = tile_coord(d, lt);
   tile_tables(d, tc, tid, tb);
   __syncthreads();
+  WriteItems<T, VW> first;
+  tile_write_setup<T, VW, UPD>(d, tc, tid, B, tb, first);
   tile_read<T, VR>(d, tc, tid, A, lds, tb);
   __syncthreads();
-  tile_write<T, S, NC, VW, UPD, CC, SH, !UPD>(d, tc, tid, B, lds, tb, co);
+  tile_write<T, S, NC, VW, UPD, CC, SH, !UPD>(d, tc, tid, B, lds, tb, co, &first);
   __syncthreads();
 }
 

@@ -320,60 +320,95 @@ HPTTX_HD void tile_read(const TileDesc &d, const TileCoord &tc, uint32_t tid,
 
 // Write pass: item k = (row, v), row = o*TR + r; VW elements of B's run
 // c0*VS + [0, clen*VS) gathered from LDS column r, combined with alpha/beta,
-// written as one vector.
-template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT>
-HPTTX_HD void tile_write(const TileDesc &d, const TileCoord &tc, uint32_t tid,
-                         T *HPTTX_RESTRICT B, const T *lds, const TileTabs &tb,
-                         const Coef<S> &k) {
+// written as one vector.  Split in two so that a thread's first kUnroll items
+// can be set up -- and, when B is read, B's values loaded -- before the read
+// pass, overlapping the B-read latency with the A-read latency.
+template <typename T, int VW>
+struct WriteItems {
+  T y[kUnroll][VW];        // B's current values (UPDATE only)
+  int64_t boff[kUnroll];   // B element offset of the vector
+  uint32_t lbase[kUnroll]; // LDS offset of (o, c = 0, r, s = 0)
+  uint32_t q0[kUnroll];    // first element of the vector within the run
+  bool ok[kUnroll];
+};
+
+template <typename T, int VW, bool UPD>
+HPTTX_HD void tile_write_setup(const TileDesc &d, const TileCoord &tc, uint32_t k0,
+                               const T *HPTTX_RESTRICT B, const TileTabs &tb,
+                               WriteItems<T, VW> &w) {
   const uint32_t nvw = d.nvw;
   const uint32_t items = nvw * d.TR * tc.olen;
   const uint32_t run = tc.clen * d.VS;
   const int64_t bbase = (int64_t)tc.c0 * d.VS;
-  for (uint32_t k0 = tid; k0 < items; k0 += kBlock * kUnroll) {
-    T x[kUnroll][VW];
-    T y[kUnroll][VW];
-    int64_t boff[kUnroll];
-    bool ok[kUnroll];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const uint32_t kk = k0 + (uint32_t)u * kBlock;
-      const uint32_t row = fdiv(kk, d.div_nvw);
-      const uint32_t v = kk - row * nvw;
-      const uint32_t o = fdiv(row, d.div_TR);
-      const uint32_t r = row - o * d.TR;
-      ok[u] = (kk < items) && (r < tc.rlen) && (v * VW < run);
-      boff[u] = 0;
-      if (ok[u]) {
-        boff[u] = bbase + v * VW + tb.BR[r] + tb.BO[o];
-        const uint32_t lrow0 = o * d.TC;
-#pragma unroll
-        for (int j = 0; j < VW; ++j) {
-          const uint32_t q = v * VW + j;
-          uint32_t c = q, s = 0;
-          if (SH) {
-            c = fdiv(q, d.div_VS);
-            s = q - c * d.VS;
-          }
-          const T *src = lds + (lrow0 + c) * d.lds_row + r * d.VS + s;
-          HPTTX_ACCESS(src, sizeof(T), false);
-          x[u][j] = *src;
-        }
-        if (UPD) vload<T, VW>(B + boff[u], y[u]);
-      }
+  for (int u = 0; u < kUnroll; ++u) {
+    const uint32_t kk = k0 + (uint32_t)u * kBlock;
+    const uint32_t row = fdiv(kk, d.div_nvw);
+    const uint32_t v = kk - row * nvw;
+    const uint32_t o = fdiv(row, d.div_TR);
+    const uint32_t r = row - o * d.TR;
+    w.ok[u] = (kk < items) && (r < tc.rlen) && (v * VW < run);
+    w.boff[u] = 0;
+    w.q0[u] = v * VW;
+    w.lbase[u] = o * d.TC * d.lds_row + r * d.VS;
+    if (w.ok[u]) {
+      w.boff[u] = bbase + v * VW + tb.BR[r] + tb.BO[o];
+      if (UPD) vload<T, VW>(B + w.boff[u], w.y[u]);
     }
+  }
+}
+
+template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT>
+HPTTX_HD void tile_write_finish(const TileDesc &d, T *HPTTX_RESTRICT B, const T *lds,
+                                const WriteItems<T, VW> &w, const Coef<S> &k) {
+  T x[kUnroll][VW];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      if (!ok[u]) continue;
-      T out[VW];
+  for (int u = 0; u < kUnroll; ++u) {
+    if (!w.ok[u]) continue;
 #pragma unroll
-      for (int j = 0; j < VW; ++j) {
-        const Elem<S, NC> &xe = reinterpret_cast<const Elem<S, NC> &>(x[u][j]);
-        const Elem<S, NC> &ye = reinterpret_cast<const Elem<S, NC> &>(y[u][j]);
-        Elem<S, NC> re = axpby<S, NC, UPD, CC>(xe, ye, k);
-        __builtin_memcpy(&out[j], &re, sizeof(T));
+    for (int j = 0; j < VW; ++j) {
+      const uint32_t q = w.q0[u] + j;
+      uint32_t c = q, s = 0;
+      if (SH) {
+        c = fdiv(q, d.div_VS);
+        s = q - c * d.VS;
       }
-      vstore<T, VW, NT>(B + boff[u], out);
+      const T *src = lds + w.lbase[u] + c * d.lds_row + s;
+      HPTTX_ACCESS(src, sizeof(T), false);
+      x[u][j] = *src;
     }
+  }
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    if (!w.ok[u]) continue;
+    T out[VW];
+#pragma unroll
+    for (int j = 0; j < VW; ++j) {
+      const Elem<S, NC> &xe = reinterpret_cast<const Elem<S, NC> &>(x[u][j]);
+      const Elem<S, NC> &ye = reinterpret_cast<const Elem<S, NC> &>(w.y[u][j]);
+      Elem<S, NC> re = axpby<S, NC, UPD, CC>(xe, ye, k);
+      __builtin_memcpy(&out[j], &re, sizeof(T));
+    }
+    vstore<T, VW, NT>(B + w.boff[u], out);
+  }
+}
+
+// The whole write pass; `first` holds the thread's first kUnroll items already
+// set up by tile_write_setup(k0 = tid) (with B prefetched), or is null.
+template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT>
+HPTTX_HD void tile_write(const TileDesc &d, const TileCoord &tc, uint32_t tid,
+                         T *HPTTX_RESTRICT B, const T *lds, const TileTabs &tb,
+                         const Coef<S> &k, const WriteItems<T, VW> *first) {
+  const uint32_t items = d.nvw * d.TR * tc.olen;
+  uint32_t k0 = tid;
+  if (first) {
+    tile_write_finish<T, S, NC, VW, UPD, CC, SH, NT>(d, B, lds, *first, k);
+    k0 += kBlock * kUnroll;
+  }
+  for (; k0 < items; k0 += kBlock * kUnroll) {
+    WriteItems<T, VW> w;
+    tile_write_setup<T, VW, UPD>(d, tc, k0, B, tb, w);
+    tile_write_finish<T, S, NC, VW, UPD, CC, SH, NT>(d, B, lds, w, k);
   }
 }
 

@@ -20,38 +20,40 @@
 
 namespace hpttx {
 
+// One block per tile on a 2-D grid (t = y * gridDim.x + x).  The write pass's
+// first items are set up, and B prefetched, before the read pass.
 template <typename T, typename S, int NC, int VR, int VW, bool UPD, bool CC, bool SH>
 __global__ __launch_bounds__(kBlock) void k_tile(TileDesc d, const T *__restrict__ A,
                                                  T *__restrict__ B, Coef<S> co) {
   extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t t = blockIdx.y * gridDim.x + blockIdx.x;
+  if (t >= d.ntiles) return;
   T *lds = reinterpret_cast<T *>(smem);
   const TileTabs tb = tile_tabs(d, smem, sizeof(T));
   const uint32_t tid = threadIdx.x;
-  for (uint32_t t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
-    const TileCoord tc = tile_coord(d, t);
-    tile_tables(d, tc, tid, tb);
-    __syncthreads();
-    tile_read<T, VR>(d, tc, tid, A, lds, tb);
-    __syncthreads();
-    tile_write<T, S, NC, VW, UPD, CC, SH, !UPD>(d, tc, tid, B, lds, tb, co);
-    __syncthreads();
-  }
+  const TileCoord tc = tile_coord(d, t);
+  tile_tables(d, tc, tid, tb);
+  __syncthreads();
+  WriteItems<T, VW> first;
+  tile_write_setup<T, VW, UPD>(d, tc, tid, B, tb, first);
+  tile_read<T, VR>(d, tc, tid, A, lds, tb);
+  __syncthreads();
+  tile_write<T, S, NC, VW, UPD, CC, SH, !UPD>(d, tc, tid, B, lds, tb, co, &first);
 }
 
 template <typename T, typename S, int NC, int V, bool UPD, bool CC>
 __global__ __launch_bounds__(kBlock) void k_rows(RowDesc d, const T *__restrict__ A,
                                                  T *__restrict__ B, Coef<S> co) {
   extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t t = blockIdx.y * gridDim.x + blockIdx.x;
+  if (t >= d.ntiles) return;
   int64_t *tabA = reinterpret_cast<int64_t *>(smem);
   int64_t *tabB = tabA + d.RB;
   const uint32_t tid = threadIdx.x;
-  for (uint32_t t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
-    const RowCoord rc = row_coord(d, t);
-    row_tables(d, rc, tid, tabA, tabB);
-    __syncthreads();
-    row_body<T, S, NC, V, UPD, CC, !UPD>(d, rc, tid, A, B, tabA, tabB, co);
-    __syncthreads();
-  }
+  const RowCoord rc = row_coord(d, t);
+  row_tables(d, rc, tid, tabA, tabB);
+  __syncthreads();
+  row_body<T, S, NC, V, UPD, CC, !UPD>(d, rc, tid, A, B, tabA, tabB, co);
 }
 
 namespace {
@@ -61,8 +63,10 @@ hipError_t launch_typed(const LaunchPlan &lp, const void *A, void *B,
                         const Coef<S> &co, hipStream_t st) {
   const T *a = static_cast<const T *>(A);
   T *b = static_cast<T *>(B);
-  dim3 grid(lp.grid), block(kBlock);
   if (lp.grid == 0) return hipSuccess;
+  // 2-D grid: x * 256 threads must stay below 2^32 work items per dimension.
+  const uint32_t gx = lp.grid < kGridX ? lp.grid : kGridX;
+  dim3 grid(gx, (lp.grid + gx - 1) / gx), block(kBlock);
   if (lp.kind == kTile) {
     const TileDesc &d = lp.tile;
     const bool sh = d.VS > 1;

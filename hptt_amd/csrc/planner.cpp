@@ -348,14 +348,14 @@ void finalize_vectors(const Problem &p, LaunchPlan &lp, uint32_t vr, uint32_t vw
     t.lds_row = row + vr;
     if (vr == 1 && (row % 2) == 0) t.lds_row = row + 1;
     lp.lds_bytes = tile_lds_bytes(t, p.esize);
-    lp.grid = std::min<uint32_t>(t.ntiles, kMaxGrid);
+    lp.grid = t.ntiles;
   } else {
     RowDesc &r = lp.rows;
     r.v = vr;
     r.nvs = (r.SEG + vr - 1) / vr;
     r.div_nvs = make_fastdiv(r.nvs);
     lp.lds_bytes = row_lds_bytes(r);
-    lp.grid = std::min<uint32_t>(r.ntiles, kMaxGrid);
+    lp.grid = r.ntiles;
   }
   (void)vw;
 }

@@ -89,8 +89,9 @@ void finalize_vectors(const Problem &p, LaunchPlan &lp, uint32_t vr, uint32_t vw
 // Widest vector (elements) a pointer's alignment allows for this element size.
 uint32_t align_vec(const void *ptr, uint32_t esize, uint32_t cap);
 
-// Maximum number of blocks one launch uses (larger tile counts loop).
-constexpr uint32_t kMaxGrid = 1u << 20;
+// Blocks per grid row; a launch covers its tiles with ceil(tiles / kGridX)
+// rows, one block per tile.
+constexpr uint32_t kGridX = 1u << 20;
 
 std::string describe(const Problem &p, const LaunchPlan &lp);
 

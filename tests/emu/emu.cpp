@@ -65,9 +65,14 @@ void run_tile(const LaunchPlan &lp, const T *A, T *B, const Coef<S> &k) {
   for (uint32_t t = 0; t < d.ntiles; ++t) {
     const TileCoord tc = tile_coord(d, t);
     for (uint32_t tid = 0; tid < kBlock; ++tid) tile_tables(d, tc, tid, tb);
-    for (uint32_t tid = 0; tid < kBlock; ++tid) tile_read<T, VR>(d, tc, tid, A, lds, tb);
+    // per-thread state kept across the barrier, as in the kernel
+    std::vector<WriteItems<T, VW>> first(kBlock);
+    for (uint32_t tid = 0; tid < kBlock; ++tid) {
+      tile_write_setup<T, VW, UPD>(d, tc, tid, B, tb, first[tid]);
+      tile_read<T, VR>(d, tc, tid, A, lds, tb);
+    }
     for (uint32_t tid = 0; tid < kBlock; ++tid)
-      tile_write<T, S, NC, VW, UPD, CC, SH, !UPD>(d, tc, tid, B, lds, tb, k);
+      tile_write<T, S, NC, VW, UPD, CC, SH, !UPD>(d, tc, tid, B, lds, tb, k, &first[tid]);
     if (!emu::g_fault.empty()) return;
   }
 }
