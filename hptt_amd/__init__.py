@@ -23,7 +23,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libhptt_amd.so")
+# HPTTX_LIB selects another build of the library (e.g. one compiled with a
+# different streaming policy); the default is the in-tree build.
+LIB_PATH = os.environ.get("HPTTX_LIB") or os.path.join(_HERE, "libhptt_amd.so")
 
 OK, EINVAL, ENOMEM, EHIP, EUNSUPPORTED = 0, 1, 2, 3, 4
 F32, F64, C64, C128 = 0, 1, 2, 3

@@ -36,9 +36,9 @@ def sources():
     return out
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC] + COMMON
+def _compile(src: str, build_dir: str = BUILD, defines=()) -> str:
+    obj = os.path.join(build_dir, os.path.basename(src) + ".o")
+    cmd = [HIPCC] + COMMON + ["-D" + d for d in defines]
     if src.endswith(".hip"):
         cmd += DEVICE
     cmd += ["-c", src, "-o", obj]
@@ -48,24 +48,34 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose: bool = False, defines=(), out: str = LIB) -> str:
+    """Compile every source and link `out`.  `defines` (NAME=VALUE) select
+    compile-time policies, e.g. HPTTX_NT_LOAD=0 for ordinary loads."""
+    build_dir = BUILD if not defines else BUILD + "_" + "_".join(
+        d.replace("=", "") for d in defines)
+    os.makedirs(build_dir, exist_ok=True)
     srcs = sources()
     jobs = min(len(srcs), int(os.environ.get("MAX_JOBS", "8")), 16)
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda s: _compile(s, build_dir, defines), srcs))
+    LIBOUT = out
+    tmp = LIBOUT + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + [
         "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
-    os.replace(tmp, LIB)
+    os.replace(tmp, LIBOUT)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", LIBOUT)
+    return LIBOUT
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-D", dest="defines", action="append", default=[])
+    ap.add_argument("--out", default=LIB)
+    a = ap.parse_args()
+    build(verbose=True, defines=a.defines, out=a.out)
     sys.exit(0)

@@ -16,9 +16,11 @@ namespace hpttx {
 
 namespace {
 
-constexpr uint32_t kVarRows = 1, kVarVr = 2, kVarVw = 4, kVarUpd = 8, kVarSh = 16, kVarCc = 32;
+constexpr uint32_t kVarRows = 1, kVarVr = 2, kVarVw = 4, kVarUpd = 8, kVarSh = 16, kVarCc = 32,
+                   kVarNt = 64;
+constexpr int kVarCodes = 128;
 
-template <typename T, typename S, int NC, int VR, int VW, bool UPD, bool CC, bool SH>
+template <typename T, typename S, int NC, int VR, int VW, bool UPD, bool CC, bool SH, bool NT>
 __device__ __forceinline__ void tile_one(const TileDesc &d, uint32_t lt, uint32_t tid,
                                          const T *A, T *B, unsigned char *smem,
                                          const Coef<S> &co) {
@@ -27,15 +29,15 @@ __device__ __forceinline__ void tile_one(const TileDesc &d, uint32_t lt, uint32_
   const TileCoord tc = tile_coord(d, lt);
   tile_tables(d, tc, tid, tb);
   __syncthreads();
-  WriteItems<T, VW> first;
-  tile_write_setup<T, VW, UPD>(d, tc, tid, B, tb, first);
-  tile_read<T, VR>(d, tc, tid, A, lds, tb);
+  WriteItems<T, VW, kUcapBatch> first;
+  tile_write_setup<T, VW, UPD, NT, kUcapBatch>(d, tc, tid, B, tb, first);
+  tile_read<T, VR, NT, kUcapBatch>(d, tc, tid, A, lds, tb);
   __syncthreads();
-  tile_write<T, S, NC, VW, UPD, CC, SH, !UPD>(d, tc, tid, B, lds, tb, co, &first);
+  tile_write<T, S, NC, VW, UPD, CC, SH, NT, kUcapBatch>(d, tc, tid, B, lds, tb, co, &first);
   __syncthreads();
 }
 
-template <typename T, typename S, int NC, int V, bool UPD, bool CC>
+template <typename T, typename S, int NC, int V, bool UPD, bool CC, bool NT>
 __device__ __forceinline__ void rows_one(const RowDesc &d, uint32_t lt, uint32_t tid,
                                          const T *A, T *B, unsigned char *smem,
                                          const Coef<S> &co) {
@@ -44,7 +46,7 @@ __device__ __forceinline__ void rows_one(const RowDesc &d, uint32_t lt, uint32_t
   const RowCoord rc = row_coord(d, lt);
   row_tables(d, rc, tid, tabA, tabB);
   __syncthreads();
-  row_body<T, S, NC, V, UPD, CC, !UPD>(d, rc, tid, A, B, tabA, tabB, co);
+  row_body<T, S, NC, V, UPD, CC, NT, kUcapBatch>(d, rc, tid, A, B, tabA, tabB, co);
   __syncthreads();
 }
 
@@ -66,19 +68,20 @@ __device__ __forceinline__ void run_code(const BatchEntry &e, uint32_t lt, uint3
   constexpr bool upd = CODE & kVarUpd;
   constexpr bool sh = CODE & kVarSh;
   constexpr bool cc = CODE & kVarCc;
+  constexpr bool nt = CODE & kVarNt;
   const Coef<S> co{(S)e.ar, (S)e.ai, (S)e.br, (S)e.bi};
   const T *A = static_cast<const T *>(e.A);
   T *B = static_cast<T *>(e.B);
   if constexpr (rows)
-    rows_one<T, S, NC, vr, upd, cc>(e.u.rows, lt, tid, A, B, smem, co);
+    rows_one<T, S, NC, vr, upd, cc, nt>(e.u.rows, lt, tid, A, B, smem, co);
   else
-    tile_one<T, S, NC, vr, vw, upd, cc, sh>(e.u.tile, lt, tid, A, B, smem, co);
+    tile_one<T, S, NC, vr, vw, upd, cc, sh, nt>(e.u.tile, lt, tid, A, B, smem, co);
 }
 
 template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
 __device__ __forceinline__ void dispatch(uint32_t v, const BatchEntry &e, uint32_t lt,
                                          uint32_t tid, unsigned char *smem) {
-  if constexpr (CODE < 64) {
+  if constexpr (CODE < kVarCodes) {
     if constexpr (valid_code<VCAP, CCOK>(CODE)) {
       if (v == CODE) {
         run_code<T, S, NC, VCAP, CCOK, CODE>(e, lt, tid, smem);
@@ -181,6 +184,7 @@ uint32_t entry_variant(const LaunchPlan &lp, bool update, bool cc, uint32_t vcap
   }
   if (update) v |= kVarUpd;
   if (cc) v |= kVarCc;
+  if (lp.stream) v |= kVarNt;
   return v;
 }
 

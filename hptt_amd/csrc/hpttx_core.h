@@ -42,7 +42,15 @@ namespace hpttx {
 
 constexpr int kMaxRank = 6;
 constexpr uint32_t kBlock = 256;   // threads per block (4 wave64)
-constexpr int kUnroll = 4;         // independent vector accesses per thread per step
+// Cap on accesses in flight per thread for narrow (sub-16-byte) accesses:
+// single-plan kernels afford 8; the batch kernel, which holds every variant's
+// registers at once, keeps 4.
+constexpr int kUcapPlan = 8;
+constexpr int kUcapBatch = 4;
+// Streaming mode (a per-plan choice): loads and stores of A and B carry the
+// non-temporal hint.  It pays when every access is part of a long run (a
+// plain copy gains ~10% on gfx950); with short runs, lines shared by
+// neighbouring tiles are better kept in cache, so plans choose per shape.
 
 enum Dtype : int { kF32 = 0, kF64 = 1, kC64 = 2, kC128 = 3 };
 
@@ -154,11 +162,16 @@ template <> struct RawVec<4> { typedef uint32_t type; };
 template <> struct RawVec<8> { typedef u32x2 type; };
 template <> struct RawVec<16> { typedef u32x4 type; };
 
-template <typename T, int V>
+template <typename T, int V, bool NT = false>
 HPTTX_HD void vload(const T *p, T (&out)[V]) {
   typedef typename RawVec<sizeof(T) * V>::type R;
   HPTTX_ACCESS(p, sizeof(R), false);
+#if defined(__HIP_DEVICE_COMPILE__)
+  R r = NT ? __builtin_nontemporal_load(reinterpret_cast<const R *>(p))
+           : *reinterpret_cast<const R *>(p);
+#else
   R r = *reinterpret_cast<const R *>(p);
+#endif
   __builtin_memcpy(&out[0], &r, sizeof(R));
 }
 
@@ -288,21 +301,32 @@ HPTTX_HD void tile_tables(const TileDesc &d, const TileCoord &tc, uint32_t tid,
   }
 }
 
+// Independent accesses a thread keeps in flight per step: 4 x 16-byte
+// vectors, and up to 8 narrower accesses (32 bytes of scalars), so that
+// shapes that only allow narrow accesses keep more bytes in flight without
+// giving up occupancy.
+template <typename T, int V, int CAP>
+constexpr int unroll_for() {
+  return (64 / (int)(V * sizeof(T))) < 4 ? 4
+       : (64 / (int)(V * sizeof(T))) > CAP ? CAP : (64 / (int)(V * sizeof(T)));
+}
+
 // Read pass: item k = (row, v), row = o*TC + c; one vector of VR elements
 // from A's run r0*VS + [0, rlen*VS) into LDS row `row`.
-template <typename T, int VR>
+template <typename T, int VR, bool NT, int UCAP>
 HPTTX_HD void tile_read(const TileDesc &d, const TileCoord &tc, uint32_t tid,
                         const T *HPTTX_RESTRICT A, T *lds, const TileTabs &tb) {
+  constexpr int U = unroll_for<T, VR, UCAP>();
   const uint32_t nvr = d.nvr;
   const uint32_t items = nvr * d.TC * tc.olen;
   const uint32_t run = tc.rlen * d.VS;
   const int64_t abase = (int64_t)tc.r0 * d.VS;
-  for (uint32_t k0 = tid; k0 < items; k0 += kBlock * kUnroll) {
-    T buf[kUnroll][VR];
-    uint32_t lidx[kUnroll];
-    bool ok[kUnroll];
+  for (uint32_t k0 = tid; k0 < items; k0 += kBlock * U) {
+    T buf[U][VR];
+    uint32_t lidx[U];
+    bool ok[U];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
+    for (int u = 0; u < U; ++u) {
       const uint32_t k = k0 + (uint32_t)u * kBlock;
       const uint32_t row = fdiv(k, d.div_nvr);
       const uint32_t v = k - row * nvr;
@@ -310,105 +334,115 @@ HPTTX_HD void tile_read(const TileDesc &d, const TileCoord &tc, uint32_t tid,
       const uint32_t c = row - o * d.TC;
       ok[u] = (k < items) && (c < tc.clen) && (v * VR < run);
       lidx[u] = row * d.lds_row + v * VR;
-      if (ok[u]) vload<T, VR>(A + abase + v * VR + tb.AC[c] + tb.AO[o], buf[u]);
+      if (ok[u]) vload<T, VR, NT>(A + abase + v * VR + tb.AC[c] + tb.AO[o], buf[u]);
     }
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u)
+    for (int u = 0; u < U; ++u)
       if (ok[u]) vstore<T, VR, false>(lds + lidx[u], buf[u]);
   }
 }
 
 // Write pass: item k = (row, v), row = o*TR + r; VW elements of B's run
 // c0*VS + [0, clen*VS) gathered from LDS column r, combined with alpha/beta,
-// written as one vector.  Split in two so that a thread's first kUnroll items
-// can be set up -- and, when B is read, B's values loaded -- before the read
-// pass, overlapping the B-read latency with the A-read latency.
-template <typename T, int VW>
+// written as one vector.  Split in two so that a thread's first U items can
+// be set up -- and, when B is read, B's values loaded -- before the read pass,
+// overlapping the B-read latency with the A-read latency.  Only the item
+// index and B's values are kept between the halves; addresses are recomputed.
+template <typename T, int VW, int UCAP>
 struct WriteItems {
-  T y[kUnroll][VW];        // B's current values (UPDATE only)
-  int64_t boff[kUnroll];   // B element offset of the vector
-  uint32_t lbase[kUnroll]; // LDS offset of (o, c = 0, r, s = 0)
-  uint32_t q0[kUnroll];    // first element of the vector within the run
-  bool ok[kUnroll];
+  static constexpr int U = unroll_for<T, VW, UCAP>();
+  T y[U][VW];       // B's current values (UPDATE only)
+  uint32_t k0;      // first item; item u is k0 + u*kBlock
 };
 
-template <typename T, int VW, bool UPD>
-HPTTX_HD void tile_write_setup(const TileDesc &d, const TileCoord &tc, uint32_t k0,
-                               const T *HPTTX_RESTRICT B, const TileTabs &tb,
-                               WriteItems<T, VW> &w) {
+struct WriteAddr {
+  int64_t boff;     // B element offset of the vector
+  uint32_t lbase;   // LDS offset of (o, c = 0, r, s = 0)
+  uint32_t q0;      // first element of the vector within the run
+  bool ok;
+};
+
+HPTTX_HD WriteAddr write_addr(const TileDesc &d, const TileCoord &tc, uint32_t kk,
+                              uint32_t vw, const TileTabs &tb) {
   const uint32_t nvw = d.nvw;
   const uint32_t items = nvw * d.TR * tc.olen;
-  const uint32_t run = tc.clen * d.VS;
-  const int64_t bbase = (int64_t)tc.c0 * d.VS;
+  const uint32_t row = fdiv(kk, d.div_nvw);
+  const uint32_t v = kk - row * nvw;
+  const uint32_t o = fdiv(row, d.div_TR);
+  const uint32_t r = row - o * d.TR;
+  WriteAddr w;
+  w.ok = (kk < items) && (r < tc.rlen) && (v * vw < tc.clen * d.VS);
+  w.q0 = v * vw;
+  w.lbase = o * d.TC * d.lds_row + r * d.VS;
+  w.boff = w.ok ? (int64_t)tc.c0 * d.VS + v * vw + tb.BR[r] + tb.BO[o] : 0;
+  return w;
+}
+
+template <typename T, int VW, bool UPD, bool NT, int UCAP>
+HPTTX_HD void tile_write_setup(const TileDesc &d, const TileCoord &tc, uint32_t k0,
+                               const T *HPTTX_RESTRICT B, const TileTabs &tb,
+                               WriteItems<T, VW, UCAP> &w) {
+  constexpr int U = WriteItems<T, VW, UCAP>::U;
+  w.k0 = k0;
+  if (!UPD) return;
 #pragma unroll
-  for (int u = 0; u < kUnroll; ++u) {
-    const uint32_t kk = k0 + (uint32_t)u * kBlock;
-    const uint32_t row = fdiv(kk, d.div_nvw);
-    const uint32_t v = kk - row * nvw;
-    const uint32_t o = fdiv(row, d.div_TR);
-    const uint32_t r = row - o * d.TR;
-    w.ok[u] = (kk < items) && (r < tc.rlen) && (v * VW < run);
-    w.boff[u] = 0;
-    w.q0[u] = v * VW;
-    w.lbase[u] = o * d.TC * d.lds_row + r * d.VS;
-    if (w.ok[u]) {
-      w.boff[u] = bbase + v * VW + tb.BR[r] + tb.BO[o];
-      if (UPD) vload<T, VW>(B + w.boff[u], w.y[u]);
-    }
+  for (int u = 0; u < U; ++u) {
+    const WriteAddr a = write_addr(d, tc, k0 + (uint32_t)u * kBlock, VW, tb);
+    if (a.ok) vload<T, VW, NT>(B + a.boff, w.y[u]);
   }
 }
 
-template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT>
-HPTTX_HD void tile_write_finish(const TileDesc &d, T *HPTTX_RESTRICT B, const T *lds,
-                                const WriteItems<T, VW> &w, const Coef<S> &k) {
-  T x[kUnroll][VW];
+template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT, int UCAP>
+HPTTX_HD void tile_write_finish(const TileDesc &d, const TileCoord &tc, T *HPTTX_RESTRICT B,
+                                const T *lds, const TileTabs &tb,
+                                const WriteItems<T, VW, UCAP> &w, const Coef<S> &k) {
+  constexpr int U = WriteItems<T, VW, UCAP>::U;
 #pragma unroll
-  for (int u = 0; u < kUnroll; ++u) {
-    if (!w.ok[u]) continue;
+  for (int u = 0; u < U; ++u) {
+    const WriteAddr a = write_addr(d, tc, w.k0 + (uint32_t)u * kBlock, VW, tb);
+    if (!a.ok) continue;
+    T x[VW];
 #pragma unroll
     for (int j = 0; j < VW; ++j) {
-      const uint32_t q = w.q0[u] + j;
+      const uint32_t q = a.q0 + j;
       uint32_t c = q, s = 0;
       if (SH) {
         c = fdiv(q, d.div_VS);
         s = q - c * d.VS;
       }
-      const T *src = lds + w.lbase[u] + c * d.lds_row + s;
+      const T *src = lds + a.lbase + c * d.lds_row + s;
       HPTTX_ACCESS(src, sizeof(T), false);
-      x[u][j] = *src;
+      x[j] = *src;
     }
-  }
-#pragma unroll
-  for (int u = 0; u < kUnroll; ++u) {
-    if (!w.ok[u]) continue;
     T out[VW];
 #pragma unroll
     for (int j = 0; j < VW; ++j) {
-      const Elem<S, NC> &xe = reinterpret_cast<const Elem<S, NC> &>(x[u][j]);
+      const Elem<S, NC> &xe = reinterpret_cast<const Elem<S, NC> &>(x[j]);
       const Elem<S, NC> &ye = reinterpret_cast<const Elem<S, NC> &>(w.y[u][j]);
       Elem<S, NC> re = axpby<S, NC, UPD, CC>(xe, ye, k);
       __builtin_memcpy(&out[j], &re, sizeof(T));
     }
-    vstore<T, VW, NT>(B + w.boff[u], out);
+    vstore<T, VW, NT>(B + a.boff, out);
   }
 }
 
-// The whole write pass; `first` holds the thread's first kUnroll items already
-// set up by tile_write_setup(k0 = tid) (with B prefetched), or is null.
-template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT>
+// The whole write pass; `first` holds the thread's first items already set up
+// by tile_write_setup(k0 = tid) (with B prefetched), or is null.
+template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT, int UCAP>
 HPTTX_HD void tile_write(const TileDesc &d, const TileCoord &tc, uint32_t tid,
                          T *HPTTX_RESTRICT B, const T *lds, const TileTabs &tb,
-                         const Coef<S> &k, const WriteItems<T, VW> *first) {
+                         const Coef<S> &k, const WriteItems<T, VW, UCAP> *first) {
+  constexpr int U = WriteItems<T, VW, UCAP>::U;
   const uint32_t items = d.nvw * d.TR * tc.olen;
   uint32_t k0 = tid;
   if (first) {
-    tile_write_finish<T, S, NC, VW, UPD, CC, SH, NT>(d, B, lds, *first, k);
-    k0 += kBlock * kUnroll;
+    tile_write_finish<T, S, NC, VW, UPD, CC, SH, NT, UCAP>(d, tc, B, lds, tb, *first, k);
+    k0 += kBlock * U;
   }
-  for (; k0 < items; k0 += kBlock * kUnroll) {
-    WriteItems<T, VW> w;
-    tile_write_setup<T, VW, UPD>(d, tc, k0, B, tb, w);
-    tile_write_finish<T, S, NC, VW, UPD, CC, SH, NT>(d, B, lds, w, k);
+  for (; k0 < items; k0 += kBlock * U) {
+    WriteItems<T, VW, UCAP> w;
+    tile_write_setup<T, VW, UPD, NT, UCAP>(d, tc, k0, B, tb, w);
+    tile_write_finish<T, S, NC, VW, UPD, CC, SH, NT, UCAP>(d, tc, B, lds, tb, w, k);
   }
 }
 
@@ -441,35 +475,36 @@ HPTTX_HD void row_tables(const RowDesc &d, const RowCoord &rc, uint32_t tid,
   }
 }
 
-template <typename T, typename S, int NC, int V, bool UPD, bool CC, bool NT>
+template <typename T, typename S, int NC, int V, bool UPD, bool CC, bool NT, int UCAP>
 HPTTX_HD void row_body(const RowDesc &d, const RowCoord &rc, uint32_t tid,
                        const T *HPTTX_RESTRICT A, T *HPTTX_RESTRICT B,
                        const int64_t *tabA, const int64_t *tabB, const Coef<S> &k) {
+  constexpr int U = unroll_for<T, V, UCAP>();
   const uint32_t nvs = d.nvs;
   const uint32_t items = nvs * rc.nr;
-  for (uint32_t k0 = tid; k0 < items; k0 += kBlock * kUnroll) {
-    T x[kUnroll][V];
-    T y[kUnroll][V];
-    int64_t boff[kUnroll];
-    bool ok[kUnroll];
+  for (uint32_t k0 = tid; k0 < items; k0 += kBlock * U) {
+    T x[U][V];
+    T y[U][V];
+    bool ok[U];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
+    for (int u = 0; u < U; ++u) {
       const uint32_t kk = k0 + (uint32_t)u * kBlock;
       const uint32_t rr = fdiv(kk, d.div_nvs);
       const uint32_t v = kk - rr * nvs;
       ok[u] = (kk < items) && (v * V < rc.slen);
-      boff[u] = 0;
       if (ok[u]) {
         const int64_t e = (int64_t)rc.seg0 + (int64_t)v * V;
-        boff[u] = tabB[rr] + e * d.rb;
-        const int64_t aoff = tabA[rr] + e * d.ra;
-        vload<T, V>(A + aoff, x[u]);
-        if (UPD) vload<T, V>(B + boff[u], y[u]);
+        vload<T, V, NT>(A + tabA[rr] + e * d.ra, x[u]);
+        if (UPD) vload<T, V, NT>(B + tabB[rr] + e * d.rb, y[u]);
       }
     }
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
+    for (int u = 0; u < U; ++u) {
       if (!ok[u]) continue;
+      const uint32_t kk = k0 + (uint32_t)u * kBlock;
+      const uint32_t rr = fdiv(kk, d.div_nvs);
+      const uint32_t v = kk - rr * nvs;
+      const int64_t e = (int64_t)rc.seg0 + (int64_t)v * V;
       T out[V];
 #pragma unroll
       for (int j = 0; j < V; ++j) {
@@ -478,7 +513,7 @@ HPTTX_HD void row_body(const RowDesc &d, const RowCoord &rc, uint32_t tid,
         Elem<S, NC> re = axpby<S, NC, UPD, CC>(xe, ye, k);
         __builtin_memcpy(&out[j], &re, sizeof(T));
       }
-      vstore<T, V, NT>(B + boff[u], out);
+      vstore<T, V, NT>(B + tabB[rr] + e * d.rb, out);
     }
   }
 }

@@ -11,8 +11,9 @@
 //       (reference MacroTransLinear, macro_kernel_trans.cc:148-195,
 //       kernel_trans_avx2.cc:480-530), the identity/axpby case and a strided
 //       gather for layouts without a unit-stride dim.
-// Non-temporal stores are used whenever B is not read (beta below threshold),
-// as the reference streams stores in that case (macro_kernel_trans.cc:60-62).
+// Loads and stores carry the non-temporal hint (the reference streams its
+// stores only when B is not read, macro_kernel_trans.cc:60-62; on gfx950 the
+// hint pays for loads and stores alike).
 #include <hip/hip_runtime.h>
 
 #include "hpttx_core.h"
@@ -22,7 +23,7 @@ namespace hpttx {
 
 // One block per tile on a 2-D grid (t = y * gridDim.x + x).  The write pass's
 // first items are set up, and B prefetched, before the read pass.
-template <typename T, typename S, int NC, int VR, int VW, bool UPD, bool CC, bool SH>
+template <typename T, typename S, int NC, int VR, int VW, bool UPD, bool CC, bool SH, bool NT>
 __global__ __launch_bounds__(kBlock) void k_tile(TileDesc d, const T *__restrict__ A,
                                                  T *__restrict__ B, Coef<S> co) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -34,14 +35,14 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileDesc d, const T *__restrict
   const TileCoord tc = tile_coord(d, t);
   tile_tables(d, tc, tid, tb);
   __syncthreads();
-  WriteItems<T, VW> first;
-  tile_write_setup<T, VW, UPD>(d, tc, tid, B, tb, first);
-  tile_read<T, VR>(d, tc, tid, A, lds, tb);
+  WriteItems<T, VW, kUcapPlan> first;
+  tile_write_setup<T, VW, UPD, NT, kUcapPlan>(d, tc, tid, B, tb, first);
+  tile_read<T, VR, NT, kUcapPlan>(d, tc, tid, A, lds, tb);
   __syncthreads();
-  tile_write<T, S, NC, VW, UPD, CC, SH, !UPD>(d, tc, tid, B, lds, tb, co, &first);
+  tile_write<T, S, NC, VW, UPD, CC, SH, NT, kUcapPlan>(d, tc, tid, B, lds, tb, co, &first);
 }
 
-template <typename T, typename S, int NC, int V, bool UPD, bool CC>
+template <typename T, typename S, int NC, int V, bool UPD, bool CC, bool NT>
 __global__ __launch_bounds__(kBlock) void k_rows(RowDesc d, const T *__restrict__ A,
                                                  T *__restrict__ B, Coef<S> co) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -53,12 +54,12 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowDesc d, const T *__restrict_
   const RowCoord rc = row_coord(d, t);
   row_tables(d, rc, tid, tabA, tabB);
   __syncthreads();
-  row_body<T, S, NC, V, UPD, CC, !UPD>(d, rc, tid, A, B, tabA, tabB, co);
+  row_body<T, S, NC, V, UPD, CC, NT, kUcapPlan>(d, rc, tid, A, B, tabA, tabB, co);
 }
 
 namespace {
 
-template <typename T, typename S, int NC, int VCAP, bool UPD, bool CC>
+template <typename T, typename S, int NC, int VCAP, bool UPD, bool CC, bool NT>
 hipError_t launch_typed(const LaunchPlan &lp, const void *A, void *B,
                         const Coef<S> &co, hipStream_t st) {
   const T *a = static_cast<const T *>(A);
@@ -73,7 +74,7 @@ hipError_t launch_typed(const LaunchPlan &lp, const void *A, void *B,
     const bool vr = d.vr == VCAP && VCAP > 1;
     const bool vw = d.vw == VCAP && VCAP > 1;
 #define HPTTX_TILE(VR_, VW_, SH_)                                                    \
-  hipLaunchKernelGGL((k_tile<T, S, NC, VR_, VW_, UPD, CC, SH_>), grid, block,      \
+  hipLaunchKernelGGL((k_tile<T, S, NC, VR_, VW_, UPD, CC, SH_, NT>), grid, block,  \
                      lp.lds_bytes, st, d, a, b, co)
     if (sh) {
       if (vr && vw) HPTTX_TILE(VCAP, VCAP, true);
@@ -90,25 +91,32 @@ hipError_t launch_typed(const LaunchPlan &lp, const void *A, void *B,
   } else {
     const RowDesc &d = lp.rows;
     if (d.v == VCAP && VCAP > 1)
-      hipLaunchKernelGGL((k_rows<T, S, NC, VCAP, UPD, CC>), grid, block, lp.lds_bytes,
+      hipLaunchKernelGGL((k_rows<T, S, NC, VCAP, UPD, CC, NT>), grid, block, lp.lds_bytes,
                          st, d, a, b, co);
     else
-      hipLaunchKernelGGL((k_rows<T, S, NC, 1, UPD, CC>), grid, block, lp.lds_bytes, st,
+      hipLaunchKernelGGL((k_rows<T, S, NC, 1, UPD, CC, NT>), grid, block, lp.lds_bytes, st,
                          d, a, b, co);
   }
   return hipGetLastError();
+}
+
+template <typename T, typename S, int NC, int VCAP, bool CCOK, bool NT>
+hipError_t launch_cc(const LaunchPlan &lp, const void *A, void *B, const Coef<S> &co,
+                     bool upd, bool cc, hipStream_t st) {
+  if (cc && CCOK) {
+    if (upd) return launch_typed<T, S, NC, VCAP, true, CCOK, NT>(lp, A, B, co, st);
+    return launch_typed<T, S, NC, VCAP, false, CCOK, NT>(lp, A, B, co, st);
+  }
+  if (upd) return launch_typed<T, S, NC, VCAP, true, false, NT>(lp, A, B, co, st);
+  return launch_typed<T, S, NC, VCAP, false, false, NT>(lp, A, B, co, st);
 }
 
 template <typename T, typename S, int NC, int VCAP, bool CCOK>
 hipError_t launch_upd(const LaunchPlan &lp, const void *A, void *B, const CoefD &c,
                       bool upd, bool cc, hipStream_t st) {
   Coef<S> co{(S)c.ar, (S)c.ai, (S)c.br, (S)c.bi};
-  if (cc && CCOK) {
-    if (upd) return launch_typed<T, S, NC, VCAP, true, CCOK>(lp, A, B, co, st);
-    return launch_typed<T, S, NC, VCAP, false, CCOK>(lp, A, B, co, st);
-  }
-  if (upd) return launch_typed<T, S, NC, VCAP, true, false>(lp, A, B, co, st);
-  return launch_typed<T, S, NC, VCAP, false, false>(lp, A, B, co, st);
+  if (lp.stream) return launch_cc<T, S, NC, VCAP, CCOK, true>(lp, A, B, co, upd, cc, st);
+  return launch_cc<T, S, NC, VCAP, CCOK, false>(lp, A, B, co, upd, cc, st);
 }
 
 }  // namespace

@@ -300,19 +300,44 @@ static int plan_tile(const Problem &p, bool shared, const TuneParams &tp,
   };
   while (vr > 1 && !ok_r(vr)) vr >>= 1;
   while (vw > 1 && !ok_w(vw)) vw >>= 1;
-  // Tile extents: TR*VS ~ run_r, TC*VS ~ run_c, multiples of the vectors.
-  uint32_t TR = (uint32_t)std::min<uint64_t>(VR, std::max<uint32_t>(1, tp.run_r / VS));
-  uint32_t TC = (uint32_t)std::min<uint64_t>(VC, std::max<uint32_t>(1, tp.run_c / VS));
+  // Tile extents.  Start from runs of ~run_r / run_c elements along A / B,
+  // then, while the tile holds less than half of block_elems, lengthen the
+  // shorter run (doubling) as far as its super-dim allows; the O count fills
+  // the rest.  Each extent is then balanced over its tile count (no sliver
+  // edge tiles) and kept a multiple of the vector width where the shape
+  // allows full vectors.
+  auto cdiv = [](uint64_t a, uint64_t b) { return (a + b - 1) / b; };
+  uint64_t TR = std::min<uint64_t>(VR, std::max<uint64_t>(1, cdiv(tp.run_r, VS)));
+  uint64_t TC = std::min<uint64_t>(VC, std::max<uint64_t>(1, cdiv(tp.run_c, VS)));
+  const uint64_t half_block = std::max<uint64_t>(1, tp.block_elems / 2);
+  while (TR * TC * VS < half_block) {
+    const bool can_r = TR < VR, can_c = TC < VC;
+    if (!can_r && !can_c) break;
+    const bool grow_r = can_r && (!can_c || TR <= TC);
+    if (grow_r) TR = std::min<uint64_t>(VR, TR * 2);
+    else TC = std::min<uint64_t>(VC, TC * 2);
+  }
+  auto balance = [&](uint64_t V, uint64_t T, uint32_t vec) {
+    const uint64_t n = cdiv(V, T);
+    uint64_t b = cdiv(V, n);
+    // vector granularity in index units: runs are index*VS elements
+    uint64_t g = 1;
+    while (g < vec && ((g * VS) % vec) != 0) ++g;
+    if ((VS * V) % vec == 0 && g > 1) b = std::min<uint64_t>(V, cdiv(b, g) * g);
+    return std::max<uint64_t>(1, b);
+  };
+  TR = balance(VR, TR, vr);
+  TC = balance(VC, TC, vw);
   // Whole-tile runs must stay multiples of the vector widths.
-  while (vr > 1 && ((uint64_t)TR * VS) % vr) vr >>= 1;
-  while (vw > 1 && ((uint64_t)TC * VS) % vw) vw >>= 1;
-  uint64_t per_o = (uint64_t)TR * VS * TC;
-  uint32_t TO = (uint32_t)std::max<uint64_t>(1, tp.block_elems / std::max<uint64_t>(per_o, 1));
-  TO = (uint32_t)std::min<uint64_t>(TO, VO);
-  TO = std::min<uint32_t>(TO, 512);
-  t.TR = TR;
-  t.TC = TC;
-  t.TO = TO;
+  while (vr > 1 && (TR * VS) % vr) vr >>= 1;
+  while (vw > 1 && (TC * VS) % vw) vw >>= 1;
+  const uint64_t per_o = TR * VS * TC;
+  uint64_t TO = std::max<uint64_t>(1, tp.block_elems / std::max<uint64_t>(per_o, 1));
+  TO = std::min<uint64_t>(std::min<uint64_t>(TO, VO), 512);
+  TO = cdiv(VO, cdiv(VO, TO));
+  t.TR = (uint32_t)TR;
+  t.TC = (uint32_t)TC;
+  t.TO = (uint32_t)TO;
   t.nR = (uint32_t)((VR + TR - 1) / TR);
   t.nC = (uint32_t)((VC + TC - 1) / TC);
   t.nO = (uint32_t)((VO + TO - 1) / TO);
@@ -386,6 +411,24 @@ int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
     rc = plan_tile(p, false, tp, lp, err);
   }
   if (rc != kOk) return rc;
+  // Streaming mode.  Model: on when the problem cannot stay in the 256 MiB
+  // last-level cache anyway and every access belongs to a run of at least
+  // 256 bytes on both sides (16-byte elements on the TILE path measured
+  // faster without it); the tuner may override.
+  if (tp.stream == 1) {
+    lp.stream = true;
+  } else if (tp.stream == 2) {
+    lp.stream = false;
+  } else {
+    const uint64_t bytes = p.N * p.esize * 2;
+    bool long_runs;
+    if (lp.kind == kTile)
+      long_runs = (uint64_t)lp.tile.TR * lp.tile.VS * p.esize >= 256 &&
+                  (uint64_t)lp.tile.TC * lp.tile.VS * p.esize >= 256 && p.esize < 16;
+    else
+      long_runs = lp.rows.ra == 1 && lp.rows.rb == 1 && (uint64_t)lp.rows.L * p.esize >= 256;
+    lp.stream = long_runs && bytes > (256ull << 20);
+  }
   finalize_vectors(p, lp, lp.vr_shape, lp.vw_shape);
   if (lp.lds_bytes > 65536) { err = "tile needs more than 64 KiB of LDS"; return kEUnsupported; }
   return kOk;
@@ -410,9 +453,9 @@ std::string describe(const Problem &p, const LaunchPlan &lp) {
     os << "\nrows: L=" << r.L << " stride(a,b)=" << r.ra << "," << r.rb << " rows=" << r.nrows
        << " seg=" << r.SEG << " rows/block=" << r.RB << " vec=" << r.v << " tiles=" << r.ntiles;
   }
-  os << "\ngrid: " << lp.grid << " x " << kBlock << "  tune(run_r,run_c,block,wide)="
-     << lp.tune.run_r << "," << lp.tune.run_c << "," << lp.tune.block_elems << ","
-     << lp.tune.wide_min_bytes << "\n";
+  os << "\ngrid: " << lp.grid << " x " << kBlock << "  streaming: " << (lp.stream ? "on" : "off")
+     << "  tune(run_r,run_c,block,wide,stream)=" << lp.tune.run_r << "," << lp.tune.run_c << ","
+     << lp.tune.block_elems << "," << lp.tune.wide_min_bytes << "," << lp.tune.stream << "\n";
   return os.str();
 }
 

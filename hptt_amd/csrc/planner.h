@@ -21,9 +21,10 @@ struct TuneParams {
   uint32_t run_c = 0;        // target write-run length along B, in elements
   uint32_t block_elems = 0;  // target elements per block tile
   uint32_t wide_min_bytes = 0;  // stride-1-kept rows at least this long use ROWS
+  uint32_t stream = 0;       // non-temporal loads/stores: 0 = model, 1 = on, 2 = off
   bool operator==(const TuneParams &o) const {
     return run_r == o.run_r && run_c == o.run_c && block_elems == o.block_elems &&
-           wide_min_bytes == o.wide_min_bytes;
+           wide_min_bytes == o.wide_min_bytes && stream == o.stream;
   }
 };
 
@@ -58,6 +59,7 @@ struct LaunchPlan {
   uint32_t vr_shape = 1, vw_shape = 1;  // widest vectors the strides allow
   uint32_t grid = 0;
   size_t lds_bytes = 0;
+  bool stream = false;         // non-temporal loads and stores
   TuneParams tune{};
   std::string path;            // "tile", "tile-shared", "wide-copy", "axpby", "gather"
 };

@@ -66,13 +66,13 @@ void run_tile(const LaunchPlan &lp, const T *A, T *B, const Coef<S> &k) {
     const TileCoord tc = tile_coord(d, t);
     for (uint32_t tid = 0; tid < kBlock; ++tid) tile_tables(d, tc, tid, tb);
     // per-thread state kept across the barrier, as in the kernel
-    std::vector<WriteItems<T, VW>> first(kBlock);
+    std::vector<WriteItems<T, VW, kUcapPlan>> first(kBlock);
     for (uint32_t tid = 0; tid < kBlock; ++tid) {
-      tile_write_setup<T, VW, UPD>(d, tc, tid, B, tb, first[tid]);
-      tile_read<T, VR>(d, tc, tid, A, lds, tb);
+      tile_write_setup<T, VW, UPD, false, kUcapPlan>(d, tc, tid, B, tb, first[tid]);
+      tile_read<T, VR, false, kUcapPlan>(d, tc, tid, A, lds, tb);
     }
     for (uint32_t tid = 0; tid < kBlock; ++tid)
-      tile_write<T, S, NC, VW, UPD, CC, SH, !UPD>(d, tc, tid, B, lds, tb, k, &first[tid]);
+      tile_write<T, S, NC, VW, UPD, CC, SH, false, kUcapPlan>(d, tc, tid, B, lds, tb, k, &first[tid]);
     if (!emu::g_fault.empty()) return;
   }
 }
@@ -86,7 +86,7 @@ void run_rows(const LaunchPlan &lp, const T *A, T *B, const Coef<S> &k) {
     const RowCoord rc = row_coord(d, t);
     for (uint32_t tid = 0; tid < kBlock; ++tid) row_tables(d, rc, tid, tabA.data(), tabB.data());
     for (uint32_t tid = 0; tid < kBlock; ++tid)
-      row_body<T, S, NC, V, UPD, CC, !UPD>(d, rc, tid, A, B, tabA.data(), tabB.data(), k);
+      row_body<T, S, NC, V, UPD, CC, false, kUcapPlan>(d, rc, tid, A, B, tabA.data(), tabB.data(), k);
     if (!emu::g_fault.empty()) return;
   }
 }
@@ -155,7 +155,7 @@ int emu_transpose(int dtype, const void *A, uint64_t nA, void *B, uint64_t nB, u
   int rc = build_problem(dtype, rank, in_size, perm, in_outer, out_outer, p, err);
   if (rc) { put(errbuf, errlen, err); return rc; }
   TuneParams tp{};
-  if (tune) tp = TuneParams{tune[0], tune[1], tune[2], tune[3]};
+  if (tune) tp = TuneParams{tune[0], tune[1], tune[2], tune[3], 0};
   LaunchPlan lp;
   rc = plan_launch(p, tp, lp, err);
   if (rc) { put(errbuf, errlen, err); return rc; }
@@ -228,7 +228,7 @@ int emu_plan(int dtype, uint32_t rank, const uint32_t *in_size, const uint32_t *
   int rc = build_problem(dtype, rank, in_size, perm, in_outer, out_outer, p, err);
   if (rc) { put(errbuf, errlen, err); return rc; }
   TuneParams tp{};
-  if (tune) tp = TuneParams{tune[0], tune[1], tune[2], tune[3]};
+  if (tune) tp = TuneParams{tune[0], tune[1], tune[2], tune[3], 0};
   LaunchPlan lp;
   rc = plan_launch(p, tp, lp, err);
   if (rc) { put(errbuf, errlen, err); return rc; }
