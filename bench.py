@@ -206,6 +206,145 @@ def time_batched(dtype_name, reps=5, seq_reps=3):
     return res
 
 
+def serving_trace(count, seed=5150):
+    """Serving requests: total elements log-uniform in [1K, 64M], rank 2..6
+    with dims splitting that count, random perm, dtype uniform over the four,
+    alpha/beta uniform in [-2, 2] (complex for the complex dtypes), beta = 0
+    for 25%."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = float(np.exp(rng.uniform(np.log(1024), np.log(64 * 2**20))))
+        rank = int(rng.integers(2, 7))
+        w = rng.dirichlet(np.ones(rank))
+        dims = [max(1, int(round(n ** wi))) for wi in w]
+        while int(np.prod(dims)) > 64 * 2**20:
+            dims[int(np.argmax(dims))] -= 1
+        code = int(rng.integers(0, 4))
+        cplx = code >= 2
+        al = complex(rng.uniform(-2, 2), rng.uniform(-2, 2)) if cplx else float(rng.uniform(-2, 2))
+        if rng.random() < 0.25:
+            be = 0.0
+        else:
+            be = complex(rng.uniform(-2, 2), rng.uniform(-2, 2)) if cplx else float(rng.uniform(-2, 2))
+        out.append({"code": code, "in_size": dims, "perm": [int(v) for v in rng.permutation(rank)],
+                    "alpha": al, "beta": be, "n": int(np.prod(dims))})
+    return out
+
+
+def time_serving(count=600, rates=(0.5, 1.0, 2.0), inflight_gb=8.0):
+    """Serving workload: sequential plain exec() (create plan, exec, wait, one
+    request at a time, in arrival order) against the continuous-batching
+    server, closed loop (all requests at once: capacity) and open loop with
+    Poisson arrivals at 0.5x, 1x and 2x the sequential capacity."""
+    import torch
+
+    import hptt_amd
+    trace = serving_trace(count)
+    tdt = {0: torch.float32, 1: torch.float64, 2: torch.complex64, 3: torch.complex128}
+    real = {0: torch.float32, 1: torch.float64, 2: torch.float32, 3: torch.float64}
+    g = torch.Generator(device="cuda").manual_seed(55)
+    maxn = max(r["n"] for r in trace)
+    pools_a, pools_b, offs = {}, {}, []
+    per_code = {c: 0 for c in range(4)}
+    for r in trace:
+        offs.append(per_code[r["code"]])
+        per_code[r["code"]] += (r["n"] + 63) // 64 * 64
+    for c in range(4):
+        k = 2 if c >= 2 else 1
+        pools_a[c] = torch.empty(maxn * k, dtype=real[c], device="cuda").uniform_(-1, 1, generator=g)
+        pools_b[c] = torch.empty(max(1, per_code[c]) * k, dtype=real[c], device="cuda").uniform_(
+            -1, 1, generator=g)
+        if c >= 2:
+            pools_a[c] = pools_a[c].view(tdt[c])
+            pools_b[c] = pools_b[c].view(tdt[c])
+    torch.cuda.synchronize()
+    items, args = [], []
+    for r, o in zip(trace, offs):
+        a = pools_a[r["code"]][:r["n"]]
+        b = pools_b[r["code"]][o:o + r["n"]]
+        items.append(hptt_amd.make_batch_item(a, b, r["in_size"], r["perm"], r["alpha"], r["beta"]))
+        args.append((a, b))
+    total_bytes = 0
+    L = hptt_amd.lib()
+    stream = torch.cuda.Stream()
+    sp = int(stream.cuda_stream)
+
+    def seq_run(arrivals):
+        """One request at a time: create the plan, exec, wait."""
+        t0 = time.perf_counter()
+        lat = []
+        for k, (r, (a, b)) in enumerate(zip(trace, args)):
+            due = t0 + arrivals[k]
+            while time.perf_counter() < due:
+                pass
+            p = hptt_amd.create_plan_checked(a, b, r["in_size"], r["perm"], r["alpha"], r["beta"],
+                                             stream=sp)
+            p.exec()
+            p.sync()
+            lat.append(time.perf_counter() - due)
+        return time.perf_counter() - t0, lat
+
+    def serve_run(arrivals):
+        srv = hptt_amd.Server(inflight_bytes=int(inflight_gb * 2**30), slots=3, max_batch=256)
+        t0 = time.perf_counter()
+        s0 = srv.now()
+        tickets = []
+        for k, it in enumerate(items):
+            due = t0 + arrivals[k]
+            while time.perf_counter() < due:
+                pass
+            tickets.append(srv.submit(it))
+        srv.drain()
+        st = srv.stats()
+        lat = []
+        lag = []
+        for k, t in enumerate(tickets):
+            c = srv.wait(t, 10.0)
+            if c.state != 2:
+                raise RuntimeError("serving request %d failed (%d)" % (k, c.status))
+            lat.append(c.t_done - (s0 + arrivals[k]))
+            lag.append(c.t_submit - (s0 + arrivals[k]))
+        wall = st.last_done - s0
+        launches = st.launches
+        srv.close()
+        return wall, lat, launches, float(np.max(lag))
+
+    def summary(wall, lat):
+        lat = np.sort(np.array(lat))
+        return {"transposes_per_s": round(len(lat) / wall, 1), "wall_ms": round(wall * 1e3, 2),
+                "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}
+
+    zero = [0.0] * len(trace)
+    seq_run(zero[:0] + zero)          # warm-up (plans, kernels)
+    seq_wall, seq_lat = seq_run(zero)
+    cap = len(trace) / seq_wall
+    serve_run(zero)                   # warm-up
+    sw, sl, launches, _ = serve_run(zero)
+    out = {"requests": len(trace),
+           "bytes": int(sum(r["n"] * [4, 8, 8, 16][r["code"]] * (2 + (r["beta"] != 0)) for r in trace)),
+           "sequential_closed": summary(seq_wall, seq_lat),
+           "served_closed": dict(summary(sw, sl), launches=launches),
+           "inflight_budget_gb": inflight_gb}
+    out["served_vs_sequential_throughput"] = round(
+        out["served_closed"]["transposes_per_s"] / out["sequential_closed"]["transposes_per_s"], 2)
+    rng = np.random.default_rng(99)
+    for f in rates:
+        lam = f * cap
+        arr = np.cumsum(rng.exponential(1.0 / lam, size=len(trace)))
+        arr -= arr[0]
+        w1, l1 = seq_run(arr)
+        w2, l2, n2, lag = serve_run(arr)
+        out["rate_%.1fx" % f] = {"offered_per_s": round(lam, 1),
+                                 "sequential": summary(w1, l1),
+                                 "served": dict(summary(w2, l2), launches=n2,
+                                                max_submit_lag_ms=round(lag * 1e3, 3))}
+    del pools_a, pools_b, items, args
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(budget_s=12.0):
     """Oracle (port of the reference's OpenMP RefTrans) on the fp32 workload."""
     import oracle
@@ -252,6 +391,10 @@ def main():
         sys.exit(1)
     import hptt_amd
     hptt_amd.lib()
+    if args.workload == "serving":
+        res = time_serving()
+        print(json.dumps({"workload": "serving", **res}), flush=True)
+        return
     if args.workload.startswith("batched-"):
         res = time_batched({"batched-f32": "float32", "batched-f64": "float64"}[args.workload],
                            reps=max(1, args.steps))
@@ -276,6 +419,14 @@ def main():
                 % (name, secondary[name]["batched_transposes_per_s"],
                    secondary[name]["sequential_transposes_per_s"], secondary[name]["speedup"],
                    secondary[name]["identical_to_sequential"]))
+        secondary["serving"] = time_serving()
+        sv = secondary["serving"]
+        log("bench: serving closed loop %s/s vs sequential %s/s (x%s); " % (
+            sv["served_closed"]["transposes_per_s"], sv["sequential_closed"]["transposes_per_s"],
+            sv["served_vs_sequential_throughput"]) + "; ".join(
+            "%s: served p50 %s p99 %s ms, seq p50 %s p99 %s ms" % (
+                k, v["served"]["p50_ms"], v["served"]["p99_ms"], v["sequential"]["p50_ms"],
+                v["sequential"]["p99_ms"]) for k, v in sv.items() if k.startswith("rate_")))
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, OpenMP)")

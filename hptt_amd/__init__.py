@@ -93,6 +93,7 @@ def lib() -> ctypes.CDLL:
     L.hpttx_version.restype = ctypes.c_int
     L.hpttx_version.argtypes = []
     _bind_batch(L)
+    _bind_server(L)
     _lib = L
     return L
 
@@ -299,10 +300,14 @@ def _bind_batch(L):
 
 def make_batch_item(in_data, out_data, in_size, perm, alpha, beta, dtype=None,
                     in_outer_size=(), out_outer_size=()) -> BatchItem:
-    """One hpttx_batch_item; in/out are device tensors or raw addresses."""
+    """One hpttx_batch_item; in/out are device tensors or raw addresses.
+    out_data=None (server requests only): the server allocates B."""
     code = dtype_code(dtype) if dtype is not None else None
     a, ca = _addr_and_dtype(in_data, code)
-    b, cb = _addr_and_dtype(out_data, code)
+    if out_data is None:
+        b, cb = 0, ca
+    else:
+        b, cb = _addr_and_dtype(out_data, code)
     if ca != cb:
         raise TypeError("A and B must have the same dtype")
     it = BatchItem()
@@ -375,6 +380,102 @@ class Batch:
         if h is not None and h.value and _lib is not None:
             _lib.hpttx_batch_destroy(h)
             self._h = ctypes.c_void_p(0)
+
+
+class ServerConfigC(ctypes.Structure):
+    _fields_ = [("inflight_bytes", ctypes.c_uint64), ("arena_bytes", ctypes.c_uint64),
+                ("slots", ctypes.c_uint32), ("max_batch", ctypes.c_uint32)]
+
+
+class Completion(ctypes.Structure):
+    """include/hptt_amd.h hpttx_completion (state 2 = done, 3 = failed)."""
+    _fields_ = [("ticket", ctypes.c_uint64), ("state", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("out", ctypes.c_void_p),
+                ("t_submit", ctypes.c_double), ("t_launch", ctypes.c_double),
+                ("t_done", ctypes.c_double), ("bytes", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("failed", ctypes.c_uint64), ("launches", ctypes.c_uint64),
+                ("bytes_done", ctypes.c_uint64), ("first_submit", ctypes.c_double),
+                ("last_done", ctypes.c_double), ("p50_latency", ctypes.c_double),
+                ("p99_latency", ctypes.c_double), ("mean_latency", ctypes.c_double),
+                ("inflight_budget", ctypes.c_uint64)]
+
+
+def _bind_server(L):
+    vp, u32, u64, d = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double
+    L.hpttx_server_create.restype = ctypes.c_int
+    L.hpttx_server_create.argtypes = [ctypes.POINTER(ServerConfigC), ctypes.POINTER(vp)]
+    L.hpttx_server_submit.restype = ctypes.c_int
+    L.hpttx_server_submit.argtypes = [vp, ctypes.POINTER(BatchItem), ctypes.POINTER(u64)]
+    L.hpttx_server_poll.restype = ctypes.c_int
+    L.hpttx_server_poll.argtypes = [vp, ctypes.POINTER(Completion), u32, ctypes.POINTER(u32)]
+    L.hpttx_server_wait.restype = ctypes.c_int
+    L.hpttx_server_wait.argtypes = [vp, u64, d, ctypes.POINTER(Completion)]
+    L.hpttx_server_drain.restype = ctypes.c_int
+    L.hpttx_server_drain.argtypes = [vp]
+    L.hpttx_server_release.restype = ctypes.c_int
+    L.hpttx_server_release.argtypes = [vp, u64]
+    L.hpttx_server_get_stats.restype = ctypes.c_int
+    L.hpttx_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+    L.hpttx_server_now.restype = d
+    L.hpttx_server_now.argtypes = [vp]
+    L.hpttx_server_destroy.restype = None
+    L.hpttx_server_destroy.argtypes = [vp]
+
+
+class Server:
+    """Continuous-batching transposition server (hpttx_server_* in
+    include/hptt_amd.h).  Requests are BatchItems; an item whose `out` is 0
+    gets its B allocated in the server's arena (beta must be 0)."""
+
+    def __init__(self, inflight_bytes: int = 0, arena_bytes: int = 0, slots: int = 3,
+                 max_batch: int = 256):
+        L = lib()
+        cfg = ServerConfigC(int(inflight_bytes), int(arena_bytes), int(slots), int(max_batch))
+        h = ctypes.c_void_p()
+        _check(L.hpttx_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, item: BatchItem) -> int:
+        t = ctypes.c_uint64()
+        _check(lib().hpttx_server_submit(self._h, ctypes.byref(item), ctypes.byref(t)))
+        return t.value
+
+    def poll(self, max_n: int = 256):
+        buf = (Completion * max_n)()
+        n = ctypes.c_uint32()
+        _check(lib().hpttx_server_poll(self._h, buf, max_n, ctypes.byref(n)))
+        return [buf[i] for i in range(n.value)]
+
+    def wait(self, ticket: int, timeout: float = -1.0) -> Completion:
+        c = Completion()
+        _check(lib().hpttx_server_wait(self._h, ticket, float(timeout), ctypes.byref(c)))
+        return c
+
+    def drain(self) -> None:
+        _check(lib().hpttx_server_drain(self._h))
+
+    def release(self, ticket: int) -> None:
+        _check(lib().hpttx_server_release(self._h, ticket))
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(lib().hpttx_server_get_stats(self._h, ctypes.byref(st)))
+        return st
+
+    def now(self) -> float:
+        return lib().hpttx_server_now(self._h)
+
+    def close(self) -> None:
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.hpttx_server_destroy(h)
+            self._h = ctypes.c_void_p(0)
+
+    __del__ = close
 
 
 def algorithmic_bytes(n_elements: int, elem_bytes: int, update: bool) -> int:

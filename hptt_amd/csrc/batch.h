@@ -4,8 +4,10 @@
 // another: src/hptt.cc:16-93 + cgraph_trans.tcc:27-48).  Each transposition
 // is planned like a single plan (planner.h); its block tiles are numbered
 // consecutively, largest transposition first, and a persistent kernel per
-// element type takes tiles from an atomic work queue.  The launches are
-// captured in a hipGraph once and replayed.
+// element type takes chunks of tiles from an atomic work queue, finding each
+// tile's transposition by a search over the entries' first tiles.  The
+// launches are captured in a hipGraph once and replayed.  The serving loop
+// (serve.h) drives the same kernel with per-launch tables.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,8 +23,9 @@ namespace hpttx {
 // Device-side record of one transposition in a batch.
 struct BatchEntry {
   uint32_t variant;      // bit0 rows, bit1 full read vec, bit2 full write vec,
-                         // bit3 update, bit4 shared group, bit5 complex coef
-  uint32_t tile_begin;   // first global tile index
+                         // bit3 update, bit4 shared group, bit5 complex coef,
+                         // bit6 streaming (non-temporal) loads and stores
+  uint32_t tile_begin;   // first tile of this entry within its launch
   uint32_t ntiles;
   uint32_t lds_bytes;
   const void *A;
@@ -47,6 +50,30 @@ struct BatchItemSpec {
   bool cc;
 };
 
+// One item planned: problem, launch plan (vectors fixed for its pointers),
+// update flag, effective complex-coefficient flag.
+struct PlannedItem {
+  Problem p;
+  LaunchPlan lp;
+  bool update = false;
+  bool cc = false;
+};
+
+int plan_item(const BatchItemSpec &it, const TuneParams &tp, PlannedItem &out, std::string &err);
+// Tile-shape class key of a planned item (dtype, fused rank, path, size, update).
+std::string class_key(const PlannedItem &q);
+// Fill a device entry for a planned item (tile_begin left to the caller).
+void make_entry(const PlannedItem &q, const BatchItemSpec &it, BatchEntry &e);
+// Persistent blocks for one launch of dtype's kernel with this LDS per block.
+uint32_t batch_grid(int dtype, uint32_t lds_bytes, uint32_t total_tiles);
+// Launch dtype's persistent kernel over entries [0, nent) whose tiles number
+// `total`; `counter` must be zero (batch_reset) when the launch starts.
+hipError_t launch_batch_group(int dtype, const BatchEntry *d_ent, const uint32_t *d_begins,
+                              uint32_t nent, uint32_t total, uint32_t lds_bytes, uint32_t grid,
+                              uint32_t *counter, hipStream_t st);
+// Zero n work counters with a kernel (capturable; no memset node).
+hipError_t batch_reset(uint32_t *counters, uint32_t n, hipStream_t st);
+
 // One element type's share of a batch.
 struct BatchGroup {
   int dtype = 0;
@@ -55,7 +82,7 @@ struct BatchGroup {
   uint32_t lds_bytes = 0;
   uint32_t grid = 0;
   BatchEntry *d_ent = nullptr;      // device table
-  uint32_t *d_tile2ent = nullptr;   // device: tile -> entry
+  uint32_t *d_begins = nullptr;     // device: first tile of each entry
   std::vector<BatchEntry> h_ent;    // host copy
   std::vector<uint32_t> order;      // entry k <- user item order[k]
   uint64_t bytes = 0;               // algorithmic bytes
@@ -73,7 +100,7 @@ struct Batch {
   std::vector<std::string> paths;   // per user item
 };
 
-// Plan every item (class tune params may be given per dtype/rank/path).
+// Plan every item (tile shapes tuned per class when tuning_timeout != 0).
 int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, double tuning_timeout,
                 Batch &b, std::string &err);
 // Enqueue: counters reset + one persistent kernel per group.

@@ -5,7 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
-#include <numeric>
+#include <mutex>
 #include <sstream>
 
 #include "batch.h"
@@ -92,17 +92,20 @@ __device__ __forceinline__ void dispatch(uint32_t v, const BatchEntry &e, uint32
   }
 }
 
+// Persistent kernel: blocks take chunks of consecutive tiles from `counter`
+// (guided self-scheduling: chunks shrink as the queue drains) until every
+// tile is taken; every block leaves the loop once the counter passes `total`.
 template <typename T, typename S, int NC, int VCAP, bool CCOK>
 __global__ __launch_bounds__(kBlock) void k_batch(const BatchEntry *__restrict__ ents,
-                                                  const uint32_t *__restrict__ t2e,
-                                                  uint32_t total, uint32_t *counter) {
+                                                  const uint32_t *__restrict__ begins,
+                                                  uint32_t nent, uint32_t total,
+                                                  uint32_t *counter) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ uint32_t s_range[2];
   const uint32_t tid = threadIdx.x;
   const uint32_t G = gridDim.x;
   for (;;) {
     if (tid == 0) {
-      // Guided self-scheduling: chunks shrink as the queue drains.
       const uint32_t cur = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const uint32_t rem = cur < total ? total - cur : 0;
       uint32_t chunk = rem / (4 * G);
@@ -116,18 +119,31 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchEntry *__restrict__
     const uint32_t t1 = __builtin_amdgcn_readfirstlane(s_range[1]);
     __syncthreads();
     if (t0 >= total) break;
+    // Entry of t0: the last entry whose first tile is <= t0.  begins[0] = 0
+    // and begins is increasing, so lo ends at that entry.
+    uint32_t lo = 0, hi = nent;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (__builtin_amdgcn_readfirstlane(begins[mid]) <= t0) lo = mid;
+      else hi = mid;
+    }
+    uint32_t ei = lo;
+    uint32_t end = (ei + 1 < nent) ? __builtin_amdgcn_readfirstlane(begins[ei + 1]) : total;
     for (uint32_t t = t0; t < t1; ++t) {
-      const uint32_t ei = __builtin_amdgcn_readfirstlane(t2e[t]);
+      while (t >= end && ei + 1 < nent) {
+        ++ei;
+        end = (ei + 1 < nent) ? __builtin_amdgcn_readfirstlane(begins[ei + 1]) : total;
+      }
       const BatchEntry &e = ents[ei];
       const uint32_t v = __builtin_amdgcn_readfirstlane(e.variant);
-      const uint32_t lt = t - e.tile_begin;
+      const uint32_t lt = t - __builtin_amdgcn_readfirstlane(begins[ei]);
       dispatch<T, S, NC, VCAP, CCOK, 0>(v, e, lt, tid, smem);
     }
   }
 }
 
 __global__ void k_reset(uint32_t *c, uint32_t n) {
-  if (threadIdx.x < n) c[threadIdx.x] = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) c[i] = 0;
 }
 
 const void *batch_kernel(int dtype) {
@@ -138,38 +154,6 @@ const void *batch_kernel(int dtype) {
     case kC128: return reinterpret_cast<const void *>(&k_batch<Elem<double, 2>, double, 2, 1, true>);
   }
   return nullptr;
-}
-
-hipError_t launch_group(const BatchGroup &g, uint32_t *counter, hipStream_t st) {
-  dim3 grid(g.grid), block(kBlock);
-  switch (g.dtype) {
-    case kF32:
-      hipLaunchKernelGGL((k_batch<Elem<float, 1>, float, 1, 4, false>), grid, block, g.lds_bytes,
-                         st, g.d_ent, g.d_tile2ent, g.total_tiles, counter);
-      break;
-    case kF64:
-      hipLaunchKernelGGL((k_batch<Elem<double, 1>, double, 1, 2, false>), grid, block,
-                         g.lds_bytes, st, g.d_ent, g.d_tile2ent, g.total_tiles, counter);
-      break;
-    case kC64:
-      hipLaunchKernelGGL((k_batch<Elem<float, 2>, float, 2, 2, true>), grid, block, g.lds_bytes,
-                         st, g.d_ent, g.d_tile2ent, g.total_tiles, counter);
-      break;
-    case kC128:
-      hipLaunchKernelGGL((k_batch<Elem<double, 2>, double, 2, 1, true>), grid, block,
-                         g.lds_bytes, st, g.d_ent, g.d_tile2ent, g.total_tiles, counter);
-      break;
-  }
-  return hipGetLastError();
-}
-
-// Tile-shape class of a problem: the autotuner's key for batches.
-std::string class_key(const Problem &p, const LaunchPlan &lp, bool update) {
-  std::ostringstream os;
-  int bucket = 0;
-  for (uint64_t n = p.N; n > 1; n >>= 2) ++bucket;
-  os << p.dtype << '|' << p.fr << '|' << lp.path << '|' << bucket << '|' << update;
-  return os.str();
 }
 
 uint32_t entry_variant(const LaunchPlan &lp, bool update, bool cc, uint32_t vcap) {
@@ -205,65 +189,144 @@ void pick_vectors(const Problem &p, LaunchPlan &lp, const void *A, const void *B
 
 }  // namespace
 
+int plan_item(const BatchItemSpec &it, const TuneParams &tp, PlannedItem &q, std::string &err) {
+  if (!it.A || !it.B) { err = "null data pointer"; return kEInval; }
+  bool padded_in = false, padded_out = false;
+  for (uint32_t d = 0; d < it.rank && d < (uint32_t)kMaxRank; ++d) {
+    padded_in |= it.in_outer[d] != 0;
+    padded_out |= it.out_outer[d] != 0;
+  }
+  int rc = build_problem(it.dtype, it.rank, it.in_size, it.perm, padded_in ? it.in_outer : nullptr,
+                         padded_out ? it.out_outer : nullptr, q.p, err);
+  if (rc != kOk) return rc;
+  q.cc = it.cc && (it.coef.ai != 0.0 || it.coef.bi != 0.0);
+  q.update = q.cc ? (std::fabs(it.coef.br) >= 1e-16 || std::fabs(it.coef.bi) >= 1e-16)
+                  : std::fabs(it.coef.br) >= 1e-16;
+  rc = plan_launch(q.p, tp, q.lp, err);
+  if (rc != kOk) return rc;
+  pick_vectors(q.p, q.lp, it.A, it.B);
+  return kOk;
+}
+
+std::string class_key(const PlannedItem &q) {
+  std::ostringstream os;
+  int bucket = 0;
+  for (uint64_t n = q.p.N; n > 1; n >>= 2) ++bucket;
+  os << q.p.dtype << '|' << q.p.fr << '|' << q.lp.path << '|' << bucket << '|' << q.update;
+  return os.str();
+}
+
+void make_entry(const PlannedItem &q, const BatchItemSpec &it, BatchEntry &e) {
+  std::memset(&e, 0, sizeof(e));
+  e.variant = entry_variant(q.lp, q.update, q.cc, 16 / q.p.esize);
+  e.ntiles = q.lp.kind == kTile ? q.lp.tile.ntiles : q.lp.rows.ntiles;
+  e.lds_bytes = (uint32_t)q.lp.lds_bytes;
+  e.A = it.A;
+  e.B = it.B;
+  e.ar = it.coef.ar;
+  e.ai = q.cc ? it.coef.ai : 0.0;
+  e.br = it.coef.br;
+  e.bi = q.cc ? it.coef.bi : 0.0;
+  if (q.lp.kind == kTile) e.u.tile = q.lp.tile;
+  else e.u.rows = q.lp.rows;
+}
+
+uint32_t batch_grid(int dtype, uint32_t lds_bytes, uint32_t total_tiles) {
+  static std::mutex mu;
+  static std::map<std::pair<int, uint32_t>, uint32_t> cache;
+  uint32_t per_gpu;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find({dtype, lds_bytes});
+    if (it != cache.end()) {
+      per_gpu = it->second;
+    } else {
+      int dev = 0, ncu = 256, occ = 1;
+      (void)hipGetDevice(&dev);
+      (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_kernel(dtype), kBlock,
+                                                       lds_bytes) != hipSuccess || occ < 1) {
+        (void)hipGetLastError();
+        occ = 1;
+      }
+      per_gpu = (uint32_t)(ncu * occ);
+      cache[{dtype, lds_bytes}] = per_gpu;
+    }
+  }
+  return std::max<uint32_t>(1, std::min<uint32_t>(per_gpu, std::max<uint32_t>(1, total_tiles)));
+}
+
+hipError_t launch_batch_group(int dtype, const BatchEntry *d_ent, const uint32_t *d_begins,
+                              uint32_t nent, uint32_t total, uint32_t lds_bytes, uint32_t grid,
+                              uint32_t *counter, hipStream_t st) {
+  if (!total || !nent) return hipSuccess;
+  dim3 g(grid), b(kBlock);
+  switch (dtype) {
+    case kF32:
+      hipLaunchKernelGGL((k_batch<Elem<float, 1>, float, 1, 4, false>), g, b, lds_bytes, st,
+                         d_ent, d_begins, nent, total, counter);
+      break;
+    case kF64:
+      hipLaunchKernelGGL((k_batch<Elem<double, 1>, double, 1, 2, false>), g, b, lds_bytes, st,
+                         d_ent, d_begins, nent, total, counter);
+      break;
+    case kC64:
+      hipLaunchKernelGGL((k_batch<Elem<float, 2>, float, 2, 2, true>), g, b, lds_bytes, st,
+                         d_ent, d_begins, nent, total, counter);
+      break;
+    case kC128:
+      hipLaunchKernelGGL((k_batch<Elem<double, 2>, double, 2, 1, true>), g, b, lds_bytes, st,
+                         d_ent, d_begins, nent, total, counter);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t batch_reset(uint32_t *counters, uint32_t n, hipStream_t st) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, counters, n);
+  return hipGetLastError();
+}
+
 int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, double tuning_timeout,
                 Batch &b, std::string &err) {
   b = Batch();
   b.count = (uint32_t)items.size();
   b.stream = stream;
   b.paths.resize(items.size());
-  struct Planned {
-    Problem p;
-    LaunchPlan lp;
-    bool update;
-    std::string key;
-  };
-  std::vector<Planned> pl(items.size());
-  std::map<std::string, TuneParams> class_tune;
+  std::vector<PlannedItem> pl(items.size());
+  std::vector<std::string> keys(items.size());
   std::map<std::string, size_t> class_rep;   // largest member of each class
   for (size_t k = 0; k < items.size(); ++k) {
-    const BatchItemSpec &it = items[k];
-    if (!it.A || !it.B) { err = "item " + std::to_string(k) + ": null data pointer"; return kEInval; }
-    bool padded_in = false, padded_out = false;
-    for (uint32_t d = 0; d < it.rank && d < (uint32_t)kMaxRank; ++d) {
-      padded_in |= it.in_outer[d] != 0;
-      padded_out |= it.out_outer[d] != 0;
-    }
-    int rc = build_problem(it.dtype, it.rank, it.in_size, it.perm, padded_in ? it.in_outer : nullptr,
-                           padded_out ? it.out_outer : nullptr, pl[k].p, err);
+    int rc = plan_item(items[k], TuneParams{}, pl[k], err);
     if (rc != kOk) { err = "item " + std::to_string(k) + ": " + err; return rc; }
-    const bool cc = it.cc && (it.coef.ai != 0.0 || it.coef.bi != 0.0);
-    pl[k].update = cc ? (std::fabs(it.coef.br) >= 1e-16 || std::fabs(it.coef.bi) >= 1e-16)
-                      : std::fabs(it.coef.br) >= 1e-16;
-    rc = plan_launch(pl[k].p, TuneParams{}, pl[k].lp, err);
-    if (rc != kOk) { err = "item " + std::to_string(k) + ": " + err; return rc; }
-    pl[k].key = class_key(pl[k].p, pl[k].lp, pl[k].update);
-    auto it2 = class_rep.find(pl[k].key);
-    if (it2 == class_rep.end() || pl[it2->second].p.N < pl[k].p.N) class_rep[pl[k].key] = k;
+    keys[k] = class_key(pl[k]);
+    auto it2 = class_rep.find(keys[k]);
+    if (it2 == class_rep.end() || pl[it2->second].p.N < pl[k].p.N) class_rep[keys[k]] = k;
   }
   // Per-class tile shapes: tuned on the class's largest member (scratch
   // buffers), or taken from the tuning cache.
   if (tuning_timeout != 0.0) {
     const double share = tuning_timeout > 0 ? tuning_timeout / std::max<size_t>(1, class_rep.size()) : -1.0;
+    std::map<std::string, TuneParams> class_tune;
     for (auto &kv : class_rep) {
-      const Planned &rep = pl[kv.second];
-      const BatchItemSpec &it = items[kv.second];
-      const bool cc = it.cc && (it.coef.ai != 0.0 || it.coef.bi != 0.0);
+      const PlannedItem &rep = pl[kv.second];
       TuneParams tp{};
       if (!tune_lookup(rep.p, rep.update, tp)) {
-        int rc = autotune(rep.p, it.coef, rep.update, cc, share, stream, tp, err);
+        int rc = autotune(rep.p, items[kv.second].coef, rep.update, rep.cc, share, stream, tp, err);
         if (rc != kOk) return rc;
       }
       class_tune[kv.first] = tp;
     }
+    for (size_t k = 0; k < items.size(); ++k) {
+      int rc = plan_item(items[k], class_tune[keys[k]], pl[k], err);
+      if (rc != kOk) { err = "item " + std::to_string(k) + ": " + err; return rc; }
+    }
   }
   std::map<int, std::vector<size_t>> by_dtype;
   for (size_t k = 0; k < items.size(); ++k) {
-    auto ct = class_tune.find(pl[k].key);
-    if (ct != class_tune.end()) {
-      int rc = plan_launch(pl[k].p, ct->second, pl[k].lp, err);
-      if (rc != kOk) return rc;
-    }
-    pick_vectors(pl[k].p, pl[k].lp, items[k].A, items[k].B);
     b.paths[k] = pl[k].lp.path;
     by_dtype[items[k].dtype].push_back(k);
   }
@@ -277,52 +340,29 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
     std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) {
       return pl[x].p.N > pl[y].p.N;
     });
-    const uint32_t vcap = 16 / dtype_esize(g.dtype);
     uint64_t tiles = 0;
+    std::vector<uint32_t> begins;
     for (size_t k : idx) {
-      const Planned &q = pl[k];
-      const BatchItemSpec &it = items[k];
-      const bool cc = it.cc && (it.coef.ai != 0.0 || it.coef.bi != 0.0);
       BatchEntry e;
-      std::memset(&e, 0, sizeof(e));
-      e.variant = entry_variant(q.lp, q.update, cc, vcap);
+      make_entry(pl[k], items[k], e);
       e.tile_begin = (uint32_t)tiles;
-      e.ntiles = q.lp.kind == kTile ? q.lp.tile.ntiles : q.lp.rows.ntiles;
-      e.lds_bytes = (uint32_t)q.lp.lds_bytes;
-      e.A = it.A;
-      e.B = it.B;
-      e.ar = it.coef.ar;
-      e.ai = cc ? it.coef.ai : 0.0;
-      e.br = it.coef.br;
-      e.bi = cc ? it.coef.bi : 0.0;
-      if (q.lp.kind == kTile) e.u.tile = q.lp.tile;
-      else e.u.rows = q.lp.rows;
+      begins.push_back((uint32_t)tiles);
       tiles += e.ntiles;
       g.lds_bytes = std::max(g.lds_bytes, e.lds_bytes);
-      g.bytes += q.p.N * q.p.esize * (uint64_t)(q.update ? 3 : 2);
+      g.bytes += pl[k].p.N * pl[k].p.esize * (uint64_t)(pl[k].update ? 3 : 2);
       g.h_ent.push_back(e);
       g.order.push_back((uint32_t)k);
     }
     if (tiles > 0xFFFFFFFFull) { err = "batch has more than 2^32 tiles"; return kEUnsupported; }
     g.nent = (uint32_t)g.h_ent.size();
     g.total_tiles = (uint32_t)tiles;
-    std::vector<uint32_t> t2e(tiles);
-    for (uint32_t k = 0; k < g.nent; ++k)
-      std::fill(t2e.begin() + g.h_ent[k].tile_begin,
-                t2e.begin() + g.h_ent[k].tile_begin + g.h_ent[k].ntiles, k);
-    int dev = 0, ncu = 256, occ = 1;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_kernel(g.dtype), kBlock,
-                                                     g.lds_bytes) != hipSuccess || occ < 1)
-      occ = 1;
-    g.grid = (uint32_t)std::min<uint64_t>(std::max<uint32_t>(1, g.total_tiles), (uint64_t)ncu * occ);
+    g.grid = batch_grid(g.dtype, g.lds_bytes, g.total_tiles);
     he = hipMalloc(&g.d_ent, sizeof(BatchEntry) * g.nent);
-    if (he == hipSuccess) he = hipMalloc(&g.d_tile2ent, sizeof(uint32_t) * std::max<uint64_t>(1, tiles));
+    if (he == hipSuccess) he = hipMalloc(&g.d_begins, sizeof(uint32_t) * g.nent);
     if (he == hipSuccess)
       he = hipMemcpy(g.d_ent, g.h_ent.data(), sizeof(BatchEntry) * g.nent, hipMemcpyHostToDevice);
-    if (he == hipSuccess && tiles)
-      he = hipMemcpy(g.d_tile2ent, t2e.data(), sizeof(uint32_t) * tiles, hipMemcpyHostToDevice);
+    if (he == hipSuccess)
+      he = hipMemcpy(g.d_begins, begins.data(), sizeof(uint32_t) * g.nent, hipMemcpyHostToDevice);
     b.bytes += g.bytes;
     b.groups.push_back(std::move(g));
     if (he != hipSuccess) break;
@@ -341,10 +381,12 @@ hipError_t batch_enqueue(Batch &b, hipStream_t st) {
   if (b.groups.empty()) return hipSuccess;
   // Counters are cleared by a kernel (not a memset node) so the sequence can
   // be captured in a graph and replayed.
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, b.d_counters, (uint32_t)b.groups.size());
-  hipError_t e = hipGetLastError();
-  for (size_t i = 0; i < b.groups.size() && e == hipSuccess; ++i)
-    if (b.groups[i].total_tiles) e = launch_group(b.groups[i], b.d_counters + i, st);
+  hipError_t e = batch_reset(b.d_counters, (uint32_t)b.groups.size(), st);
+  for (size_t i = 0; i < b.groups.size() && e == hipSuccess; ++i) {
+    const BatchGroup &g = b.groups[i];
+    e = launch_batch_group(g.dtype, g.d_ent, g.d_begins, g.nent, g.total_tiles, g.lds_bytes, g.grid,
+                           b.d_counters + i, st);
+  }
   return e;
 }
 
@@ -374,9 +416,9 @@ void batch_free(Batch &b) {
   if (b.graph) (void)hipGraphDestroy(b.graph);
   for (BatchGroup &g : b.groups) {
     if (g.d_ent) (void)hipFree(g.d_ent);
-    if (g.d_tile2ent) (void)hipFree(g.d_tile2ent);
+    if (g.d_begins) (void)hipFree(g.d_begins);
     g.d_ent = nullptr;
-    g.d_tile2ent = nullptr;
+    g.d_begins = nullptr;
   }
   if (b.d_counters) (void)hipFree(b.d_counters);
   b.d_counters = nullptr;

@@ -18,7 +18,53 @@ struct hpttx_batch {
 // Shares hpttx_last_error with the plan interface (capi.cpp).
 namespace hpttx {
 void set_last_error(const std::string &m);
+
+// Validate one C item and convert it.  Coefficients are held at the
+// element's precision (float for F32 / C64), as the plan interface does.
+int batch_item_to_spec(const hpttx_batch_item &it, uint32_t k, BatchItemSpec &s,
+                       bool allow_null_out) {
+  std::memset(&s, 0, sizeof(s));
+  const std::string where = "item " + std::to_string(k) + ": ";
+  if (it.dtype < HPTTX_F32 || it.dtype > HPTTX_C128) {
+    set_last_error(where + "unknown dtype");
+    return HPTTX_EINVAL;
+  }
+  if (it.rank > 6) {
+    set_last_error(where + "rank above 6 is not supported");
+    return HPTTX_EUNSUPPORTED;
+  }
+  if (!it.in || (!it.out && !allow_null_out)) {
+    set_last_error(where + "null data pointer");
+    return HPTTX_EINVAL;
+  }
+  const bool cplx_type = it.dtype == HPTTX_C64 || it.dtype == HPTTX_C128;
+  if (it.complex_coef && !cplx_type) {
+    set_last_error(where + "complex coefficients need a complex dtype");
+    return HPTTX_EINVAL;
+  }
+  s.dtype = it.dtype;
+  s.A = it.in;
+  s.B = it.out;
+  s.rank = it.rank;
+  for (uint32_t d = 0; d < it.rank; ++d) {
+    s.in_size[d] = it.in_size[d];
+    s.perm[d] = it.perm[d];
+    s.in_outer[d] = it.in_outer[d];
+    s.out_outer[d] = it.out_outer[d];
+  }
+  s.cc = it.complex_coef != 0 && cplx_type;
+  if (it.dtype == HPTTX_F32 || it.dtype == HPTTX_C64)
+    s.coef = CoefD{(double)(float)it.alpha_re, (double)(float)it.alpha_im,
+                   (double)(float)it.beta_re, (double)(float)it.beta_im};
+  else
+    s.coef = CoefD{it.alpha_re, it.alpha_im, it.beta_re, it.beta_im};
+  if (!s.cc) {
+    s.coef.ai = 0.0;
+    s.coef.bi = 0.0;
+  }
+  return HPTTX_OK;
 }
+}  // namespace hpttx
 
 extern "C" {
 
@@ -29,43 +75,8 @@ int hpttx_batch_create(uint32_t count, const hpttx_batch_item *items, double tun
   if (count && !items) { hpttx::set_last_error("null item array"); return HPTTX_EINVAL; }
   std::vector<BatchItemSpec> specs(count);
   for (uint32_t k = 0; k < count; ++k) {
-    const hpttx_batch_item &it = items[k];
-    BatchItemSpec &s = specs[k];
-    std::memset(&s, 0, sizeof(s));
-    if (it.dtype < HPTTX_F32 || it.dtype > HPTTX_C128) {
-      hpttx::set_last_error("item " + std::to_string(k) + ": unknown dtype");
-      return HPTTX_EINVAL;
-    }
-    if (it.rank > 6) {
-      hpttx::set_last_error("item " + std::to_string(k) + ": rank above 6 is not supported");
-      return HPTTX_EUNSUPPORTED;
-    }
-    s.dtype = it.dtype;
-    s.A = it.in;
-    s.B = it.out;
-    s.rank = it.rank;
-    for (uint32_t d = 0; d < it.rank; ++d) {
-      s.in_size[d] = it.in_size[d];
-      s.perm[d] = it.perm[d];
-      s.in_outer[d] = it.in_outer[d];
-      s.out_outer[d] = it.out_outer[d];
-    }
-    const bool cplx_type = it.dtype == HPTTX_C64 || it.dtype == HPTTX_C128;
-    s.cc = it.complex_coef != 0 && cplx_type;
-    if (it.complex_coef && !cplx_type) {
-      hpttx::set_last_error("item " + std::to_string(k) + ": complex coefficients need a complex dtype");
-      return HPTTX_EINVAL;
-    }
-    // Coefficients are held at the element's precision (float for F32/C64).
-    if (it.dtype == HPTTX_F32 || it.dtype == HPTTX_C64)
-      s.coef = CoefD{(double)(float)it.alpha_re, (double)(float)it.alpha_im,
-                     (double)(float)it.beta_re, (double)(float)it.beta_im};
-    else
-      s.coef = CoefD{it.alpha_re, it.alpha_im, it.beta_re, it.beta_im};
-    if (!s.cc) {
-      s.coef.ai = 0.0;
-      s.coef.bi = 0.0;
-    }
+    int rc = batch_item_to_spec(items[k], k, specs[k], false);
+    if (rc != HPTTX_OK) return rc;
   }
   hpttx_batch *h = new (std::nothrow) hpttx_batch();
   if (!h) { hpttx::set_last_error("out of host memory"); return HPTTX_ENOMEM; }

@@ -1,0 +1,472 @@
+// serve.cpp -- see serve.h.
+#include "serve.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+
+namespace hpttx {
+
+namespace {
+constexpr uint64_t kGranule = 256;
+uint64_t round_up(uint64_t x, uint64_t g) { return (x + g - 1) / g * g; }
+const auto g_t0 = std::chrono::steady_clock::now();
+}  // namespace
+
+// ---------------------------------------------------------------------------
+bool Arena::init(uint64_t bytes, std::string &err) {
+  cap_ = round_up(bytes, kGranule);
+  if (!cap_) return true;
+  if (hipMalloc(&base_, cap_) != hipSuccess) {
+    (void)hipGetLastError();
+    err = "arena allocation failed";
+    base_ = nullptr;
+    cap_ = 0;
+    return false;
+  }
+  free_.clear();
+  free_[0] = cap_;
+  used_ = 0;
+  return true;
+}
+
+void Arena::destroy() {
+  if (base_) (void)hipFree(base_);
+  base_ = nullptr;
+  cap_ = used_ = 0;
+  free_.clear();
+}
+
+bool Arena::alloc(uint64_t bytes, uint64_t &off) {
+  bytes = round_up(std::max<uint64_t>(bytes, 1), kGranule);
+  for (auto it = free_.begin(); it != free_.end(); ++it) {
+    if (it->second >= bytes) {
+      off = it->first;
+      const uint64_t rest = it->second - bytes;
+      free_.erase(it);
+      if (rest) free_[off + bytes] = rest;
+      used_ += bytes;
+      return true;
+    }
+  }
+  return false;
+}
+
+void Arena::free(uint64_t off, uint64_t bytes) {
+  bytes = round_up(std::max<uint64_t>(bytes, 1), kGranule);
+  used_ -= bytes;
+  auto it = free_.emplace(off, bytes).first;
+  auto nx = std::next(it);
+  if (nx != free_.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    free_.erase(nx);
+  }
+  if (it != free_.begin()) {
+    auto pv = std::prev(it);
+    if (pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      free_.erase(it);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+double Server::now() const {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - g_t0).count();
+}
+
+int Server::start(const ServerConfig &cfg, std::string &err) {
+  cfg_ = cfg;
+  if (cfg_.slots < 1) cfg_.slots = 1;
+  if (cfg_.slots > 8) cfg_.slots = 8;
+  if (cfg_.max_batch < 1) cfg_.max_batch = 1;
+  if (hipGetDevice(&device_) != hipSuccess) { err = "no HIP device"; return kEHip; }
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) { err = "hipMemGetInfo failed"; return kEHip; }
+  budget_ = cfg_.inflight_bytes ? cfg_.inflight_bytes : (uint64_t)fr / 4;
+  if (cfg_.arena_bytes) {
+    if (cfg_.arena_bytes > (uint64_t)fr) { err = "arena larger than free HBM"; return kENoMem; }
+    if (!arena_.init(cfg_.arena_bytes, err)) return kENoMem;
+  }
+  slots_.resize(cfg_.slots);
+  for (Slot &s : slots_) {
+    hipError_t e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMalloc(&s.d_ent, sizeof(BatchEntry) * cfg_.max_batch);
+    if (e == hipSuccess) e = hipMalloc(&s.d_begins, sizeof(uint32_t) * cfg_.max_batch);
+    if (e == hipSuccess) e = hipMalloc(&s.d_counters, sizeof(uint32_t) * 4);
+    if (e == hipSuccess) e = hipHostMalloc(&s.h_ent, sizeof(BatchEntry) * cfg_.max_batch, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(&s.h_begins, sizeof(uint32_t) * cfg_.max_batch, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      err = std::string("server slot setup: ") + hipGetErrorString(e);
+      stop();
+      return e == hipErrorOutOfMemory ? kENoMem : kEHip;
+    }
+  }
+  stop_ = false;
+  dispatcher_ = std::thread([this] { dispatch_loop(); });
+  completer_ = std::thread([this] { complete_loop(); });
+  return kOk;
+}
+
+void Server::stop() {
+  {
+    std::unique_lock<std::mutex> lk(mu_);
+    // Requests still queued are failed; launches in flight finish first.
+    while (!pending_.empty()) {
+      Request *r = pending_.front();
+      pending_.pop_front();
+      r->state = kFailed;
+      r->status = kEInval;
+      finished_.push_back(r);
+    }
+    cv_done_.wait(lk, [&] {
+      for (const Slot &s : slots_)
+        if (s.busy) return false;
+      return true;
+    });
+    stop_ = true;
+  }
+  cv_dispatch_.notify_all();
+  cv_complete_.notify_all();
+  if (dispatcher_.joinable()) dispatcher_.join();
+  if (completer_.joinable()) completer_.join();
+  for (Slot &s : slots_) {
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    if (s.d_ent) (void)hipFree(s.d_ent);
+    if (s.d_begins) (void)hipFree(s.d_begins);
+    if (s.d_counters) (void)hipFree(s.d_counters);
+    if (s.h_ent) (void)hipHostFree(s.h_ent);
+    if (s.h_begins) (void)hipHostFree(s.h_begins);
+    if (s.done) (void)hipEventDestroy(s.done);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    s = Slot();
+  }
+  slots_.clear();
+  arena_.destroy();
+  for (auto &kv : all_) delete kv.second;
+  all_.clear();
+  finished_.clear();
+}
+
+int Server::plan_cached(const BatchItemSpec &spec, PlannedItem &q, std::string &err) {
+  // Plans depend on shape, perm, padding, update / complex-coefficient flags
+  // and the pointers' 16-byte alignment, not on the coefficient values.
+  const bool cc = spec.cc && (spec.coef.ai != 0.0 || spec.coef.bi != 0.0);
+  const bool upd = cc ? (std::fabs(spec.coef.br) >= 1e-16 || std::fabs(spec.coef.bi) >= 1e-16)
+                      : std::fabs(spec.coef.br) >= 1e-16;
+  const uint32_t es = dtype_esize(spec.dtype);
+  std::ostringstream os;
+  os << spec.dtype << '|' << spec.rank << '|' << upd << '|' << cc << '|'
+     << (es ? align_vec(spec.A, es, 16 / es) : 0) << '|' << (es ? align_vec(spec.B, es, 16 / es) : 0);
+  for (uint32_t d = 0; d < spec.rank && d < (uint32_t)kMaxRank; ++d)
+    os << '|' << spec.in_size[d] << ',' << spec.perm[d] << ',' << spec.in_outer[d] << ','
+       << spec.out_outer[d];
+  const std::string key = os.str();
+  {
+    std::lock_guard<std::mutex> lk(plan_mu_);
+    auto it = plans_.find(key);
+    if (it != plans_.end()) {
+      q = it->second;
+      return kOk;
+    }
+  }
+  int rc = plan_item(spec, TuneParams{}, q, err);
+  if (rc != kOk) return rc;
+  std::lock_guard<std::mutex> lk(plan_mu_);
+  if (plans_.size() > 4096) plans_.clear();
+  plans_[key] = q;
+  return kOk;
+}
+
+int Server::submit(const BatchItemSpec &spec_in, bool server_out, uint64_t &ticket, std::string &err) {
+  BatchItemSpec spec = spec_in;
+  if (server_out) {
+    if (!arena_.capacity()) { err = "server has no output arena"; return kEInval; }
+    spec.B = arena_.base();   // stands in for a 256-byte-aligned arena slot while planning
+  }
+  PlannedItem q;
+  int rc = plan_cached(spec, q, err);
+  if (rc != kOk) return rc;
+  if (server_out && q.update) { err = "server-allocated outputs need beta below 1e-16"; return kEInval; }
+  Request *r = new Request();
+  r->spec = spec;
+  r->plan = q;
+  r->bytes = q.p.N * q.p.esize * (uint64_t)(q.update ? 3 : 2);
+  r->arena_out = server_out;
+  r->arena_len = server_out ? q.p.spanB * q.p.esize : 0;
+  if (server_out && round_up(r->arena_len, kGranule) > arena_.capacity()) {
+    delete r;
+    err = "output larger than the server's arena";
+    return kENoMem;
+  }
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    if (stop_) {
+      delete r;
+      err = "server stopped";
+      return kEInval;
+    }
+    r->ticket = next_ticket_++;
+    r->t_submit = now();
+    if (stats_.first_submit < 0) stats_.first_submit = r->t_submit;
+    stats_.submitted++;
+    all_[r->ticket] = r;
+    pending_.push_back(r);
+    ticket = r->ticket;
+  }
+  cv_dispatch_.notify_one();
+  return kOk;
+}
+
+void Server::dispatch_loop() {
+  (void)hipSetDevice(device_);
+  std::unique_lock<std::mutex> lk(mu_);
+  bool blocked = false;
+  uint64_t seen_done = 0;
+  for (;;) {
+    cv_dispatch_.wait(lk, [&] {
+      if (stop_) return true;
+      if (pending_.empty()) return false;
+      if (blocked && progress_ == seen_done) return false;
+      for (const Slot &s : slots_)
+        if (!s.busy) return true;
+      return false;
+    });
+    if (stop_) break;
+    blocked = false;
+    Slot *slot = nullptr;
+    for (Slot &s : slots_)
+      if (!s.busy) {
+        slot = &s;
+        break;
+      }
+    // Admit in arrival order while the in-flight byte budget and the arena
+    // allow; a request is always admitted when nothing else is in flight.
+    std::vector<Request *> batch;
+    uint64_t add = 0;
+    while (!pending_.empty() && batch.size() < cfg_.max_batch) {
+      Request *r = pending_.front();
+      const bool idle = inflight_ + add == 0;
+      if (!idle && inflight_ + add + r->bytes > budget_) break;
+      if (r->arena_out) {
+        uint64_t off = 0;
+        if (!arena_.alloc(r->arena_len, off)) {
+          if (idle && arena_.used() == 0) {
+            pending_.pop_front();
+            r->state = kFailed;
+            r->status = kENoMem;
+            stats_.failed++;
+            progress_++;
+            finished_.push_back(r);
+            cv_done_.notify_all();
+            continue;
+          }
+          break;
+        }
+        r->arena_off = off;
+        r->spec.B = arena_.base() + off;
+      }
+      pending_.pop_front();
+      batch.push_back(r);
+      add += r->bytes;
+    }
+    if (batch.empty()) {
+      blocked = true;
+      seen_done = progress_;
+      continue;
+    }
+    inflight_ += add;
+    slot->busy = true;
+    slot->reqs = batch;
+    slot->bytes = add;
+    slot->seq = launch_seq_++;
+    stats_.launches++;
+    lk.unlock();
+
+    // One launch per element type; within a type, largest first.
+    std::stable_sort(batch.begin(), batch.end(), [](const Request *a, const Request *b) {
+      if (a->spec.dtype != b->spec.dtype) return a->spec.dtype < b->spec.dtype;
+      return a->plan.p.N > b->plan.p.N;
+    });
+    struct Group {
+      int dtype;
+      uint32_t first, count, total, lds;
+    };
+    std::vector<Group> groups;
+    hipError_t he = hipSuccess;
+    for (uint32_t i = 0; i < batch.size(); ++i) {
+      Request *r = batch[i];
+      if (groups.empty() || groups.back().dtype != r->spec.dtype)
+        groups.push_back(Group{r->spec.dtype, i, 0, 0, 0});
+      Group &g = groups.back();
+      BatchEntry &e = slot->h_ent[i];
+      make_entry(r->plan, r->spec, e);
+      if ((uint64_t)g.total + e.ntiles > 0xFFFFFFFFull) he = hipErrorInvalidValue;
+      e.tile_begin = g.total;
+      slot->h_begins[i] = g.total;
+      g.total += e.ntiles;
+      g.lds = std::max(g.lds, e.lds_bytes);
+      g.count++;
+      r->t_launch = now();
+    }
+    const size_t n = batch.size();
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(slot->d_ent, slot->h_ent, sizeof(BatchEntry) * n, hipMemcpyHostToDevice,
+                          slot->stream);
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(slot->d_begins, slot->h_begins, sizeof(uint32_t) * n,
+                          hipMemcpyHostToDevice, slot->stream);
+    if (he == hipSuccess) he = batch_reset(slot->d_counters, (uint32_t)groups.size(), slot->stream);
+    for (size_t gi = 0; gi < groups.size() && he == hipSuccess; ++gi) {
+      const Group &g = groups[gi];
+      he = launch_batch_group(g.dtype, slot->d_ent + g.first, slot->d_begins + g.first, g.count,
+                              g.total, g.lds, batch_grid(g.dtype, g.lds, g.total),
+                              slot->d_counters + gi, slot->stream);
+    }
+    if (he == hipSuccess) he = hipEventRecord(slot->done, slot->stream);
+    lk.lock();
+    if (he != hipSuccess) {
+      // Nothing (more) was enqueued for this slot: fail its requests once the
+      // stream has drained what was.
+      lk.unlock();
+      (void)hipStreamSynchronize(slot->stream);
+      lk.lock();
+      for (Request *r : slot->reqs) {
+        r->state = kFailed;
+        r->status = kEHip;
+        stats_.failed++;
+        progress_++;
+        if (r->arena_out) arena_.free(r->arena_off, r->arena_len);
+        r->arena_out = false;
+        finished_.push_back(r);
+      }
+      inflight_ -= slot->bytes;
+      slot->reqs.clear();
+      slot->busy = false;
+      cv_done_.notify_all();
+      continue;
+    }
+    for (Request *r : slot->reqs) r->state = kRunning;
+    slot->seq |= 1ull << 63;   // launched: the completer may poll its event
+    cv_complete_.notify_one();
+  }
+}
+
+void Server::complete_loop() {
+  (void)hipSetDevice(device_);
+  for (;;) {
+    std::vector<Slot *> live;
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_complete_.wait(lk, [&] {
+        if (stop_) return true;
+        for (const Slot &s : slots_)
+          if (s.busy && (s.seq >> 63)) return true;
+        return false;
+      });
+      for (Slot &s : slots_)
+        if (s.busy && (s.seq >> 63)) live.push_back(&s);
+      if (stop_ && live.empty()) return;
+    }
+    std::vector<Slot *> done;
+    for (Slot *s : live) {
+      const hipError_t e = hipEventQuery(s->done);
+      if (e == hipSuccess) done.push_back(s);
+      else if (e != hipErrorNotReady) done.push_back(s);   // surfaced at the next sync
+    }
+    if (done.empty()) {
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+      continue;
+    }
+    const double t = now();
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (Slot *s : done) {
+        for (Request *r : s->reqs) {
+          r->t_done = t;
+          r->state = kDone;
+          stats_.completed++;
+          stats_.bytes_done += r->bytes;
+          lat_.push_back(t - r->t_submit);
+          finished_.push_back(r);
+        }
+        stats_.last_done = t;
+        progress_++;
+        inflight_ -= s->bytes;
+        s->reqs.clear();
+        s->busy = false;
+        s->seq = 0;
+      }
+    }
+    cv_dispatch_.notify_one();
+    cv_done_.notify_all();
+  }
+}
+
+size_t Server::poll(Request *out, size_t max) {
+  std::lock_guard<std::mutex> lk(mu_);
+  size_t n = 0;
+  while (n < max && !finished_.empty()) {
+    out[n++] = *finished_.front();
+    finished_.pop_front();
+  }
+  return n;
+}
+
+bool Server::wait(uint64_t ticket, double timeout_s, Request &out) {
+  std::unique_lock<std::mutex> lk(mu_);
+  auto pred = [&] {
+    auto it = all_.find(ticket);
+    return it == all_.end() || it->second->state >= kDone;
+  };
+  if (timeout_s < 0) cv_done_.wait(lk, pred);
+  else if (!cv_done_.wait_for(lk, std::chrono::duration<double>(timeout_s), pred)) return false;
+  auto it = all_.find(ticket);
+  if (it == all_.end()) return false;
+  out = *it->second;
+  return true;
+}
+
+void Server::drain() {
+  std::unique_lock<std::mutex> lk(mu_);
+  cv_done_.wait(lk, [&] {
+    if (!pending_.empty()) return false;
+    for (const Slot &s : slots_)
+      if (s.busy) return false;
+    return true;
+  });
+}
+
+int Server::release(uint64_t ticket) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = all_.find(ticket);
+  if (it == all_.end()) return kEInval;
+  Request *r = it->second;
+  if (r->state < kDone) return kEInval;
+  if (r->arena_out) arena_.free(r->arena_off, r->arena_len);
+  for (auto f = finished_.begin(); f != finished_.end(); ++f)
+    if (*f == r) {
+      finished_.erase(f);
+      break;
+    }
+  all_.erase(it);
+  delete r;
+  progress_++;
+  cv_dispatch_.notify_one();
+  return kOk;
+}
+
+ServerStats Server::stats() {
+  std::lock_guard<std::mutex> lk(mu_);
+  return stats_;
+}
+
+void Server::latencies(std::vector<double> &out) {
+  std::lock_guard<std::mutex> lk(mu_);
+  out = lat_;
+}
+
+}  // namespace hpttx

@@ -82,7 +82,7 @@ std::vector<TuneParams> tune_candidates(const Problem &p) {
   c.push_back(d);
   const uint32_t e = p.esize;
   const uint32_t runs[] = {64 / e, 128 / e, 256 / e, 512 / e};
-  const uint32_t blocks[] = {8192 / e, 16384 / e, 32768 / e};
+  const uint32_t blocks[] = {2048 / e, 4096 / e, 8192 / e, 16384 / e};
   const uint32_t wides[] = {128, 256, 1024};
   for (uint32_t rr : runs)
     for (uint32_t rc : runs)

@@ -204,6 +204,64 @@ int hpttx_batch_item_path(const hpttx_batch *batch, uint32_t index, char *buf,
                           size_t len);
 void hpttx_batch_destroy(hpttx_batch *batch);
 
+/*
+ * Serving: a continuous-batching transposition server.  No counterpart in
+ * the reference (one synchronous plan->exec() per transposition,
+ * inc/hptt/impl/hptt_trans.h:27).  Requests (hpttx_batch_item, one
+ * transposition each) are queued by hpttx_server_submit and admitted in
+ * arrival order while the algorithmic bytes in flight stay within the budget
+ * (default: a quarter of the free HBM at creation) and the output arena has
+ * room; every admitted request goes into the next launch of the persistent
+ * batch kernel on one of `slots` streams, so requests arriving while others
+ * run join the next launch and launches overlap on the device.  Plans are
+ * cached by shape.  A request whose `out` is NULL gets B allocated in the
+ * server's arena (beta must then be below 1e-16); its address is reported at
+ * completion and the buffer is held until hpttx_server_release.
+ */
+typedef struct hpttx_server hpttx_server;
+
+typedef struct hpttx_server_config {
+  uint64_t inflight_bytes;   /* 0: a quarter of the free HBM */
+  uint64_t arena_bytes;      /* bytes for server-allocated outputs (0: none) */
+  uint32_t slots;            /* launches in flight (streams), 1..8; 0: 3 */
+  uint32_t max_batch;        /* requests per launch; 0: 256 */
+} hpttx_server_config;
+
+typedef struct hpttx_completion {
+  uint64_t ticket;
+  int32_t state;             /* 2 done, 3 failed */
+  int32_t status;            /* HPTTX_* when failed */
+  void *out;                 /* B (the arena slot for server-allocated outputs) */
+  double t_submit, t_launch, t_done;   /* seconds, one steady clock */
+  uint64_t bytes;            /* algorithmic bytes */
+} hpttx_completion;
+
+typedef struct hpttx_server_stats {
+  uint64_t submitted, completed, failed, launches;
+  uint64_t bytes_done;
+  double first_submit, last_done;      /* seconds, same clock as completions */
+  double p50_latency, p99_latency, mean_latency;   /* submit -> done, seconds */
+  uint64_t inflight_budget;
+} hpttx_server_stats;
+
+int hpttx_server_create(const hpttx_server_config *config, hpttx_server **server);
+int hpttx_server_submit(hpttx_server *server, const hpttx_batch_item *request,
+                        uint64_t *ticket);
+/* Up to max finished requests not reported before; *n receives the count. */
+int hpttx_server_poll(hpttx_server *server, hpttx_completion *out,
+                      uint32_t max, uint32_t *n);
+/* Block until `ticket` has finished (timeout_s < 0: no limit). */
+int hpttx_server_wait(hpttx_server *server, uint64_t ticket, double timeout_s,
+                      hpttx_completion *out);
+/* Block until every submitted request has finished. */
+int hpttx_server_drain(hpttx_server *server);
+/* Forget a finished request and free its arena output, if any. */
+int hpttx_server_release(hpttx_server *server, uint64_t ticket);
+int hpttx_server_get_stats(hpttx_server *server, hpttx_server_stats *stats);
+/* Seconds on the clock the completion times use. */
+double hpttx_server_now(hpttx_server *server);
+void hpttx_server_destroy(hpttx_server *server);
+
 /* Reason for the last failure on this thread ("" if none). */
 const char *hpttx_last_error(void);
 int hpttx_version(void);

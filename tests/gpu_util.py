@@ -36,3 +36,15 @@ def run_plan(A: np.ndarray, B0: np.ndarray, size, perm, alpha, beta, io=None, oo
     out = host(b_view)
     del plan
     return out, info
+
+
+def read_device(addr: int, n: int, dtype) -> np.ndarray:
+    """Copy n elements at a raw device address to the host (hipMemcpy)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.restype = ctypes.c_int
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    out = np.empty(n, dtype=dtype)
+    rc = hip.hipMemcpy(out.ctypes.data, addr, out.nbytes, 2)  # device to host
+    assert rc == 0, rc
+    return out

@@ -1,0 +1,132 @@
+"""Serving mode on the GPU: requests admitted as they arrive, batched into
+persistent launches on several streams; every result must equal, bit for bit,
+the same transposition run alone through a plain plan and the oracle."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import hptt_amd
+import oracle
+import shapes
+from gpu_util import dev, host, read_device
+
+pytestmark = pytest.mark.gpu
+
+NP = {0: np.float32, 1: np.float64, 2: np.complex64, 3: np.complex128}
+
+
+def _requests(rng, count, max_elems=150000):
+    out = []
+    for k in range(count):
+        code = int(rng.integers(0, 4))
+        size, perm = shapes.random_case(rng, max_elems=max_elems, max_dim=64)
+        cc = code >= 2 and k % 3 == 0
+        al = complex(rng.uniform(-2, 2), rng.uniform(-1, 1)) if cc else float(rng.uniform(-2, 2))
+        if k % 4 == 0:
+            be = 0.0
+        else:
+            be = complex(rng.uniform(-2, 2), rng.uniform(-1, 1)) if cc else float(rng.uniform(-2, 2))
+        n = int(np.prod(size))
+        A = oracle.uniform(n, NP[code], 10 * k + 1)
+        B = oracle.uniform(n, NP[code], 10 * k + 2)
+        out.append(dict(code=code, size=size, perm=perm, alpha=al, beta=be, cc=cc, A=A, B=B,
+                        dA=dev(A), dB=dev(B)))
+    return out
+
+
+def _item(r, out=True):
+    return hptt_amd.make_batch_item(r["dA"], r["dB"] if out else None, r["size"], r["perm"],
+                                    r["alpha"], r["beta"])
+
+
+def _alone(r):
+    """The same transposition through a plain plan, on a fresh copy of B."""
+    import torch
+    dB = dev(r["B"])
+    p = hptt_amd.create_plan_checked(r["dA"], dB, r["size"], r["perm"], r["alpha"], r["beta"])
+    p.exec()
+    p.sync()
+    torch.cuda.synchronize()
+    return host(dB)
+
+
+def test_served_equals_alone_and_oracle(device):
+    """Requests arrive from a second thread while earlier ones run."""
+    import torch
+    rng = np.random.default_rng(2024)
+    reqs = _requests(rng, 160)
+    torch.cuda.synchronize()
+    srv = hptt_amd.Server(slots=3, max_batch=64)
+    tickets = [None] * len(reqs)
+
+    def client():
+        for k, r in enumerate(reqs):
+            tickets[k] = srv.submit(_item(r))
+            if k % 7 == 0:
+                time.sleep(0.0005)
+
+    th = threading.Thread(target=client)
+    th.start()
+    th.join()
+    srv.drain()
+    st = srv.stats()
+    assert st.completed == len(reqs) and st.failed == 0
+    assert 1 <= st.launches <= len(reqs)
+    assert 0 < st.p50_latency <= st.p99_latency
+    for k, r in enumerate(reqs):
+        c = srv.wait(tickets[k], 5.0)
+        assert c.state == 2, (k, c.status)
+        got = host(r["dB"])
+        exp = r["B"].copy()
+        oracle.transpose(r["A"], exp, r["size"], r["perm"], r["alpha"], r["beta"], nthreads=8,
+                         complex_coef=r["cc"])
+        assert oracle.first_mismatch(got, exp) == -1, k
+        assert oracle.first_mismatch(got, _alone(r)) == -1, k
+    srv.close()
+
+
+def test_server_allocated_outputs(device):
+    import torch
+    rng = np.random.default_rng(77)
+    reqs = _requests(rng, 40)
+    for r in reqs:
+        r["beta"] = 0.0
+    torch.cuda.synchronize()
+    srv = hptt_amd.Server(arena_bytes=64 << 20, slots=2, max_batch=16)
+    tickets = [srv.submit(_item(r, out=False)) for r in reqs]
+    srv.drain()
+    for k, (t, r) in enumerate(zip(tickets, reqs)):
+        c = srv.wait(t, 5.0)
+        assert c.state == 2 and c.out
+        got = read_device(c.out, r["A"].size, NP[r["code"]])
+        exp = np.zeros_like(r["B"])
+        oracle.transpose(r["A"], exp, r["size"], r["perm"], r["alpha"], 0.0,
+                         complex_coef=r["cc"])
+        assert oracle.first_mismatch(got, exp) == -1, k
+        srv.release(t)
+    # a beta != 0 request cannot use a server-allocated (uninitialised) B
+    bad = _item(reqs[0], out=False)
+    bad.beta_re = 1.0
+    with pytest.raises(hptt_amd.HpttError):
+        srv.submit(bad)
+    srv.close()
+
+
+def test_admission_budget_serialises(device):
+    """With an in-flight budget smaller than any request, requests still all
+    complete, each admitted alone (one launch per request)."""
+    import torch
+    rng = np.random.default_rng(5)
+    reqs = _requests(rng, 12, max_elems=50000)
+    torch.cuda.synchronize()
+    srv = hptt_amd.Server(inflight_bytes=1, slots=3)
+    tickets = [srv.submit(_item(r)) for r in reqs]
+    srv.drain()
+    st = srv.stats()
+    assert st.completed == len(reqs) and st.launches == len(reqs)
+    for t, r in zip(tickets, reqs):
+        assert srv.wait(t, 5.0).state == 2
+        assert oracle.first_mismatch(host(r["dB"]), _alone(r)) == -1
+    srv.close()
