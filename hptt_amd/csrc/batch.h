@@ -20,6 +20,9 @@
 
 namespace hpttx {
 
+// Streams a batch spreads its launches over (the exec stream plus side ones).
+constexpr size_t kBatchLanes = 4;
+
 // Device-side record of one transposition in a batch.
 struct BatchEntry {
   uint32_t variant;      // bit0 rows, bit1 full read vec, bit2 full write vec,
@@ -64,19 +67,25 @@ int plan_item(const BatchItemSpec &it, const TuneParams &tp, PlannedItem &out, s
 std::string class_key(const PlannedItem &q);
 // Fill a device entry for a planned item (tile_begin left to the caller).
 void make_entry(const PlannedItem &q, const BatchItemSpec &it, BatchEntry &e);
-// Persistent blocks for one launch of dtype's kernel with this LDS per block.
-uint32_t batch_grid(int dtype, uint32_t lds_bytes, uint32_t total_tiles);
-// Launch dtype's persistent kernel over entries [0, nent) whose tiles number
-// `total`; `counter` must be zero (batch_reset) when the launch starts.
-hipError_t launch_batch_group(int dtype, const BatchEntry *d_ent, const uint32_t *d_begins,
-                              uint32_t nent, uint32_t total, uint32_t lds_bytes, uint32_t grid,
-                              uint32_t *counter, hipStream_t st);
+// LDS per block a batch launch uses for entries needing lds_bytes (rounded
+// up to 4 KiB).
+uint32_t batch_lds_bytes(uint32_t lds_bytes);
+// Persistent blocks for one launch of the (dtype, variant) kernel with this
+// LDS per block (occupancy looked up once per kernel and LDS size, cached).
+uint32_t batch_grid(int dtype, uint32_t variant, uint32_t lds_bytes, uint32_t total_tiles);
+// Launch the persistent kernel of (dtype, variant) over entries [0, nent),
+// all of that variant, whose tiles number `total`; `counter` must be zero
+// (batch_reset) when the launch starts.
+hipError_t launch_batch_group(int dtype, uint32_t variant, const BatchEntry *d_ent,
+                              const uint32_t *d_begins, uint32_t nent, uint32_t total,
+                              uint32_t lds_bytes, uint32_t grid, uint32_t *counter, hipStream_t st);
 // Zero n work counters with a kernel (capturable; no memset node).
 hipError_t batch_reset(uint32_t *counters, uint32_t n, hipStream_t st);
 
-// One element type's share of a batch.
+// The entries of one (element type, variant): one persistent launch.
 struct BatchGroup {
   int dtype = 0;
+  uint32_t variant = 0;
   uint32_t nent = 0;
   uint32_t total_tiles = 0;
   uint32_t lds_bytes = 0;
@@ -93,6 +102,11 @@ struct Batch {
   uint32_t count = 0;
   uint32_t *d_counters = nullptr;   // one work counter per group
   hipStream_t stream = nullptr;
+  // Launches fork from the exec stream onto side streams and join back, so
+  // the groups run concurrently (also inside the captured graph).
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> events;   // [0] fork, [1 + i] join of side[i]
+  std::vector<int> lane;            // stream of each group: 0 exec, i+1 side[i]
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   bool use_graph = true;

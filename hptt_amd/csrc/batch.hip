@@ -7,6 +7,7 @@
 #include <map>
 #include <mutex>
 #include <sstream>
+#include <tuple>
 
 #include "batch.h"
 #include "hpttx_core.h"
@@ -78,24 +79,12 @@ __device__ __forceinline__ void run_code(const BatchEntry &e, uint32_t lt, uint3
     tile_one<T, S, NC, vr, vw, upd, cc, sh, nt>(e.u.tile, lt, tid, A, B, smem, co);
 }
 
+// Persistent kernel for one variant: blocks take chunks of consecutive tiles
+// from `counter` (guided self-scheduling: chunks shrink as the queue drains)
+// until every tile is taken; every block leaves the loop once the counter
+// passes `total`.  One instance per (element type, variant), so each launch
+// carries only its variant's code and registers.
 template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
-__device__ __forceinline__ void dispatch(uint32_t v, const BatchEntry &e, uint32_t lt,
-                                         uint32_t tid, unsigned char *smem) {
-  if constexpr (CODE < kVarCodes) {
-    if constexpr (valid_code<VCAP, CCOK>(CODE)) {
-      if (v == CODE) {
-        run_code<T, S, NC, VCAP, CCOK, CODE>(e, lt, tid, smem);
-        return;
-      }
-    }
-    dispatch<T, S, NC, VCAP, CCOK, CODE + 1>(v, e, lt, tid, smem);
-  }
-}
-
-// Persistent kernel: blocks take chunks of consecutive tiles from `counter`
-// (guided self-scheduling: chunks shrink as the queue drains) until every
-// tile is taken; every block leaves the loop once the counter passes `total`.
-template <typename T, typename S, int NC, int VCAP, bool CCOK>
 __global__ __launch_bounds__(kBlock) void k_batch(const BatchEntry *__restrict__ ents,
                                                   const uint32_t *__restrict__ begins,
                                                   uint32_t nent, uint32_t total,
@@ -134,10 +123,8 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchEntry *__restrict__
         ++ei;
         end = (ei + 1 < nent) ? __builtin_amdgcn_readfirstlane(begins[ei + 1]) : total;
       }
-      const BatchEntry &e = ents[ei];
-      const uint32_t v = __builtin_amdgcn_readfirstlane(e.variant);
       const uint32_t lt = t - __builtin_amdgcn_readfirstlane(begins[ei]);
-      dispatch<T, S, NC, VCAP, CCOK, 0>(v, e, lt, tid, smem);
+      run_code<T, S, NC, VCAP, CCOK, CODE>(ents[ei], lt, tid, smem);
     }
   }
 }
@@ -146,14 +133,33 @@ __global__ void k_reset(uint32_t *c, uint32_t n) {
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) c[i] = 0;
 }
 
-const void *batch_kernel(int dtype) {
-  switch (dtype) {
-    case kF32: return reinterpret_cast<const void *>(&k_batch<Elem<float, 1>, float, 1, 4, false>);
-    case kF64: return reinterpret_cast<const void *>(&k_batch<Elem<double, 1>, double, 1, 2, false>);
-    case kC64: return reinterpret_cast<const void *>(&k_batch<Elem<float, 2>, float, 2, 2, true>);
-    case kC128: return reinterpret_cast<const void *>(&k_batch<Elem<double, 2>, double, 2, 1, true>);
+// Host table of kernel instances: [dtype][variant code] (null where the code
+// does not exist for the type).
+template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
+void fill_kernels(const void **tab) {
+  if constexpr (CODE < kVarCodes) {
+    if constexpr (valid_code<VCAP, CCOK>(CODE))
+      tab[CODE] = reinterpret_cast<const void *>(&k_batch<T, S, NC, VCAP, CCOK, CODE>);
+    else
+      tab[CODE] = nullptr;
+    fill_kernels<T, S, NC, VCAP, CCOK, CODE + 1>(tab);
   }
-  return nullptr;
+}
+
+struct KernelTable {
+  const void *k[4][kVarCodes];
+  KernelTable() {
+    fill_kernels<Elem<float, 1>, float, 1, 4, false, 0>(k[kF32]);
+    fill_kernels<Elem<double, 1>, double, 1, 2, false, 0>(k[kF64]);
+    fill_kernels<Elem<float, 2>, float, 2, 2, true, 0>(k[kC64]);
+    fill_kernels<Elem<double, 2>, double, 2, 1, true, 0>(k[kC128]);
+  }
+};
+
+const void *batch_kernel(int dtype, uint32_t variant) {
+  static const KernelTable table;
+  if (dtype < kF32 || dtype > kC128 || variant >= (uint32_t)kVarCodes) return nullptr;
+  return table.k[dtype][variant];
 }
 
 uint32_t entry_variant(const LaunchPlan &lp, bool update, bool cc, uint32_t vcap) {
@@ -231,57 +237,54 @@ void make_entry(const PlannedItem &q, const BatchItemSpec &it, BatchEntry &e) {
   else e.u.rows = q.lp.rows;
 }
 
-uint32_t batch_grid(int dtype, uint32_t lds_bytes, uint32_t total_tiles) {
+uint32_t batch_lds_bytes(uint32_t lds_bytes) {
+  // 4 KiB granules: few distinct launch shapes, so occupancy is looked up
+  // (and cached) for at most 16 sizes per element type.
+  return (lds_bytes + 4095u) & ~4095u;
+}
+
+uint32_t batch_grid(int dtype, uint32_t variant, uint32_t lds_bytes, uint32_t total_tiles) {
   static std::mutex mu;
-  static std::map<std::pair<int, uint32_t>, uint32_t> cache;
+  static std::map<std::tuple<int, uint32_t, uint32_t>, uint32_t> cache;
+  static int ncu = 0;
   uint32_t per_gpu;
+  lds_bytes = batch_lds_bytes(lds_bytes);
   {
     std::lock_guard<std::mutex> lk(mu);
-    auto it = cache.find({dtype, lds_bytes});
+    if (!ncu) {
+      int dev = 0;
+      (void)hipGetDevice(&dev);
+      if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+          ncu < 1)
+        ncu = 256;
+    }
+    auto key = std::make_tuple(dtype, variant, lds_bytes);
+    auto it = cache.find(key);
     if (it != cache.end()) {
       per_gpu = it->second;
     } else {
-      int dev = 0, ncu = 256, occ = 1;
-      (void)hipGetDevice(&dev);
-      (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_kernel(dtype), kBlock,
-                                                       lds_bytes) != hipSuccess || occ < 1) {
+      int occ = 1;
+      const void *k = batch_kernel(dtype, variant);
+      if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, kBlock, lds_bytes) != hipSuccess ||
+          occ < 1) {
         (void)hipGetLastError();
         occ = 1;
       }
       per_gpu = (uint32_t)(ncu * occ);
-      cache[{dtype, lds_bytes}] = per_gpu;
+      cache[key] = per_gpu;
     }
   }
   return std::max<uint32_t>(1, std::min<uint32_t>(per_gpu, std::max<uint32_t>(1, total_tiles)));
 }
 
-hipError_t launch_batch_group(int dtype, const BatchEntry *d_ent, const uint32_t *d_begins,
-                              uint32_t nent, uint32_t total, uint32_t lds_bytes, uint32_t grid,
-                              uint32_t *counter, hipStream_t st) {
+hipError_t launch_batch_group(int dtype, uint32_t variant, const BatchEntry *d_ent,
+                              const uint32_t *d_begins, uint32_t nent, uint32_t total,
+                              uint32_t lds_bytes, uint32_t grid, uint32_t *counter, hipStream_t st) {
   if (!total || !nent) return hipSuccess;
-  dim3 g(grid), b(kBlock);
-  switch (dtype) {
-    case kF32:
-      hipLaunchKernelGGL((k_batch<Elem<float, 1>, float, 1, 4, false>), g, b, lds_bytes, st,
-                         d_ent, d_begins, nent, total, counter);
-      break;
-    case kF64:
-      hipLaunchKernelGGL((k_batch<Elem<double, 1>, double, 1, 2, false>), g, b, lds_bytes, st,
-                         d_ent, d_begins, nent, total, counter);
-      break;
-    case kC64:
-      hipLaunchKernelGGL((k_batch<Elem<float, 2>, float, 2, 2, true>), g, b, lds_bytes, st,
-                         d_ent, d_begins, nent, total, counter);
-      break;
-    case kC128:
-      hipLaunchKernelGGL((k_batch<Elem<double, 2>, double, 2, 1, true>), g, b, lds_bytes, st,
-                         d_ent, d_begins, nent, total, counter);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  const void *k = batch_kernel(dtype, variant);
+  if (!k) return hipErrorInvalidValue;
+  void *args[] = {(void *)&d_ent, (void *)&d_begins, (void *)&nent, (void *)&total, (void *)&counter};
+  return hipLaunchKernel(k, dim3(grid), dim3(kBlock), args, lds_bytes, st);
 }
 
 hipError_t batch_reset(uint32_t *counters, uint32_t n, hipStream_t st) {
@@ -325,15 +328,18 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
       if (rc != kOk) { err = "item " + std::to_string(k) + ": " + err; return rc; }
     }
   }
-  std::map<int, std::vector<size_t>> by_dtype;
+  std::map<std::pair<int, uint32_t>, std::vector<size_t>> by_kernel;
   for (size_t k = 0; k < items.size(); ++k) {
     b.paths[k] = pl[k].lp.path;
-    by_dtype[items[k].dtype].push_back(k);
+    BatchEntry probe;
+    make_entry(pl[k], items[k], probe);
+    by_kernel[{items[k].dtype, probe.variant}].push_back(k);
   }
   hipError_t he = hipSuccess;
-  for (auto &kv : by_dtype) {
+  for (auto &kv : by_kernel) {
     BatchGroup g;
-    g.dtype = kv.first;
+    g.dtype = kv.first.first;
+    g.variant = kv.first.second;
     std::vector<size_t> idx = kv.second;
     // Largest transposition first: its tiles are handed out first and the
     // small ones fill the tail.
@@ -356,7 +362,8 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
     if (tiles > 0xFFFFFFFFull) { err = "batch has more than 2^32 tiles"; return kEUnsupported; }
     g.nent = (uint32_t)g.h_ent.size();
     g.total_tiles = (uint32_t)tiles;
-    g.grid = batch_grid(g.dtype, g.lds_bytes, g.total_tiles);
+    g.lds_bytes = batch_lds_bytes(g.lds_bytes);
+    g.grid = batch_grid(g.dtype, g.variant, g.lds_bytes, g.total_tiles);
     he = hipMalloc(&g.d_ent, sizeof(BatchEntry) * g.nent);
     if (he == hipSuccess) he = hipMalloc(&g.d_begins, sizeof(uint32_t) * g.nent);
     if (he == hipSuccess)
@@ -369,6 +376,34 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
   }
   if (he == hipSuccess) he = hipMalloc(&b.d_counters, sizeof(uint32_t) * std::max<size_t>(1, b.groups.size()));
   if (he == hipSuccess) he = hipMemset(b.d_counters, 0, sizeof(uint32_t) * std::max<size_t>(1, b.groups.size()));
+  // Side streams for concurrent groups; groups go, largest first, to the
+  // least loaded lane.
+  const size_t nside = std::min<size_t>(b.groups.size() > 0 ? b.groups.size() - 1 : 0, kBatchLanes - 1);
+  for (size_t i = 0; i < nside && he == hipSuccess; ++i) {
+    hipStream_t s = nullptr;
+    he = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (he == hipSuccess) b.side.push_back(s);
+  }
+  for (size_t i = 0; i < nside + 1 && he == hipSuccess; ++i) {
+    hipEvent_t e = nullptr;
+    he = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    if (he == hipSuccess) b.events.push_back(e);
+  }
+  if (he == hipSuccess) {
+    std::vector<size_t> gorder(b.groups.size());
+    for (size_t i = 0; i < gorder.size(); ++i) gorder[i] = i;
+    std::sort(gorder.begin(), gorder.end(),
+              [&](size_t x, size_t y) { return b.groups[x].bytes > b.groups[y].bytes; });
+    std::vector<uint64_t> load(nside + 1, 0);
+    b.lane.assign(b.groups.size(), 0);
+    for (size_t gi : gorder) {
+      size_t best = 0;
+      for (size_t l = 1; l < load.size(); ++l)
+        if (load[l] < load[best]) best = l;
+      b.lane[gi] = (int)best;
+      load[best] += b.groups[gi].bytes;
+    }
+  }
   if (he != hipSuccess) {
     err = std::string("batch setup: ") + hipGetErrorString(he);
     batch_free(b);
@@ -382,10 +417,18 @@ hipError_t batch_enqueue(Batch &b, hipStream_t st) {
   // Counters are cleared by a kernel (not a memset node) so the sequence can
   // be captured in a graph and replayed.
   hipError_t e = batch_reset(b.d_counters, (uint32_t)b.groups.size(), st);
+  if (e == hipSuccess && !b.side.empty()) e = hipEventRecord(b.events[0], st);
+  for (size_t i = 0; i < b.side.size() && e == hipSuccess; ++i)
+    e = hipStreamWaitEvent(b.side[i], b.events[0], 0);
   for (size_t i = 0; i < b.groups.size() && e == hipSuccess; ++i) {
     const BatchGroup &g = b.groups[i];
-    e = launch_batch_group(g.dtype, g.d_ent, g.d_begins, g.nent, g.total_tiles, g.lds_bytes, g.grid,
-                           b.d_counters + i, st);
+    hipStream_t ls = b.lane[i] == 0 ? st : b.side[b.lane[i] - 1];
+    e = launch_batch_group(g.dtype, g.variant, g.d_ent, g.d_begins, g.nent, g.total_tiles,
+                           g.lds_bytes, g.grid, b.d_counters + i, ls);
+  }
+  for (size_t i = 0; i < b.side.size() && e == hipSuccess; ++i) {
+    e = hipEventRecord(b.events[1 + i], b.side[i]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, b.events[1 + i], 0);
   }
   return e;
 }
@@ -422,6 +465,10 @@ void batch_free(Batch &b) {
   }
   if (b.d_counters) (void)hipFree(b.d_counters);
   b.d_counters = nullptr;
+  for (hipStream_t s : b.side) (void)hipStreamDestroy(s);
+  for (hipEvent_t e : b.events) (void)hipEventDestroy(e);
+  b.side.clear();
+  b.events.clear();
   b.gexec = nullptr;
   b.graph = nullptr;
   b.groups.clear();

@@ -96,7 +96,7 @@ int Server::start(const ServerConfig &cfg, std::string &err) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&s.d_ent, sizeof(BatchEntry) * cfg_.max_batch);
     if (e == hipSuccess) e = hipMalloc(&s.d_begins, sizeof(uint32_t) * cfg_.max_batch);
-    if (e == hipSuccess) e = hipMalloc(&s.d_counters, sizeof(uint32_t) * 4);
+    if (e == hipSuccess) e = hipMalloc(&s.d_counters, sizeof(uint32_t) * kMaxSlotGroups);
     if (e == hipSuccess) e = hipHostMalloc(&s.h_ent, sizeof(BatchEntry) * cfg_.max_batch, hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc(&s.h_begins, sizeof(uint32_t) * cfg_.max_batch, hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -105,6 +105,11 @@ int Server::start(const ServerConfig &cfg, std::string &err) {
       return e == hipErrorOutOfMemory ? kENoMem : kEHip;
     }
   }
+  // Look up the persistent kernels' occupancy for every LDS size a launch
+  // can use now, not on the request path.
+  for (int dt = kF32; dt <= kC128; ++dt)
+    for (uint32_t v = 0; v < 128; ++v)
+      for (uint32_t lds = 4096; lds <= 24576; lds += 4096) (void)batch_grid(dt, v, lds, 1u << 30);
   stop_ = false;
   dispatcher_ = std::thread([this] { dispatch_loop(); });
   completer_ = std::thread([this] { complete_loop(); });
@@ -286,21 +291,28 @@ void Server::dispatch_loop() {
     stats_.launches++;
     lk.unlock();
 
-    // One launch per element type; within a type, largest first.
+    // One launch per (element type, variant); within one, largest first.
+    for (Request *r : batch) {
+      BatchEntry probe;
+      make_entry(r->plan, r->spec, probe);
+      r->variant = probe.variant;
+    }
     std::stable_sort(batch.begin(), batch.end(), [](const Request *a, const Request *b) {
       if (a->spec.dtype != b->spec.dtype) return a->spec.dtype < b->spec.dtype;
+      if (a->variant != b->variant) return a->variant < b->variant;
       return a->plan.p.N > b->plan.p.N;
     });
     struct Group {
       int dtype;
-      uint32_t first, count, total, lds;
+      uint32_t variant, first, count, total, lds;
     };
     std::vector<Group> groups;
     hipError_t he = hipSuccess;
     for (uint32_t i = 0; i < batch.size(); ++i) {
       Request *r = batch[i];
-      if (groups.empty() || groups.back().dtype != r->spec.dtype)
-        groups.push_back(Group{r->spec.dtype, i, 0, 0, 0});
+      if (groups.empty() || groups.back().dtype != r->spec.dtype ||
+          groups.back().variant != r->variant)
+        groups.push_back(Group{r->spec.dtype, r->variant, i, 0, 0, 0});
       Group &g = groups.back();
       BatchEntry &e = slot->h_ent[i];
       make_entry(r->plan, r->spec, e);
@@ -308,7 +320,7 @@ void Server::dispatch_loop() {
       e.tile_begin = g.total;
       slot->h_begins[i] = g.total;
       g.total += e.ntiles;
-      g.lds = std::max(g.lds, e.lds_bytes);
+      g.lds = std::max(g.lds, batch_lds_bytes(e.lds_bytes));
       g.count++;
       r->t_launch = now();
     }
@@ -319,11 +331,12 @@ void Server::dispatch_loop() {
     if (he == hipSuccess)
       he = hipMemcpyAsync(slot->d_begins, slot->h_begins, sizeof(uint32_t) * n,
                           hipMemcpyHostToDevice, slot->stream);
+    if (groups.size() > kMaxSlotGroups) he = hipErrorInvalidValue;
     if (he == hipSuccess) he = batch_reset(slot->d_counters, (uint32_t)groups.size(), slot->stream);
     for (size_t gi = 0; gi < groups.size() && he == hipSuccess; ++gi) {
       const Group &g = groups[gi];
-      he = launch_batch_group(g.dtype, slot->d_ent + g.first, slot->d_begins + g.first, g.count,
-                              g.total, g.lds, batch_grid(g.dtype, g.lds, g.total),
+      he = launch_batch_group(g.dtype, g.variant, slot->d_ent + g.first, slot->d_begins + g.first,
+                              g.count, g.total, g.lds, batch_grid(g.dtype, g.variant, g.lds, g.total),
                               slot->d_counters + gi, slot->stream);
     }
     if (he == hipSuccess) he = hipEventRecord(slot->done, slot->stream);

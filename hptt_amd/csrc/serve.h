@@ -29,6 +29,10 @@
 
 namespace hpttx {
 
+// Distinct (element type, variant) launches one slot can hold: every variant
+// code of every type.
+constexpr size_t kMaxSlotGroups = 512;
+
 struct ServerConfig {
   uint64_t inflight_bytes = 0;   // budget of algorithmic bytes in flight (0: 1/4 of free HBM)
   uint64_t arena_bytes = 0;      // server-allocated outputs (0: none)
@@ -48,6 +52,7 @@ struct Request {
   bool arena_out = false;
   int state = kQueued;
   int status = 0;
+  uint32_t variant = 0;          // batch-kernel variant of its plan
   double t_submit = 0, t_launch = 0, t_done = 0;
 };
 
