@@ -73,19 +73,24 @@ uint32_t batch_lds_bytes(uint32_t lds_bytes);
 // Persistent blocks for one launch of the (dtype, variant) kernel with this
 // LDS per block (occupancy looked up once per kernel and LDS size, cached).
 uint32_t batch_grid(int dtype, uint32_t variant, uint32_t lds_bytes, uint32_t total_tiles);
-// Launch the persistent kernel of (dtype, variant) over entries [0, nent),
-// all of that variant, whose tiles number `total`; `counter` must be zero
-// (batch_reset) when the launch starts.
+// Launch the persistent kernel over entries [0, nent) whose tiles number
+// `total`.  `variant` is a variant code (every entry has it) or
+// batch_group_key(., true) for the instance that dispatches per entry.
 hipError_t launch_batch_group(int dtype, uint32_t variant, const BatchEntry *d_ent,
                               const uint32_t *d_begins, uint32_t nent, uint32_t total,
-                              uint32_t lds_bytes, uint32_t grid, uint32_t *counter, hipStream_t st);
-// Zero n work counters with a kernel (capturable; no memset node).
-hipError_t batch_reset(uint32_t *counters, uint32_t n, hipStream_t st);
+                              uint32_t lds_bytes, uint32_t grid, hipStream_t st);
+// Kernel key of an entry's group: its variant, or the per-type instance
+// that dispatches variants at run time when `merged`.
+uint32_t batch_group_key(uint32_t variant, bool merged);
+// Grouping used by batches and the server: one launch per element type
+// (merged, default) or one per (element type, variant) when the environment
+// variable HPTTX_BATCH_MODE is "split".
+bool batch_merged_default();
 
-// The entries of one (element type, variant): one persistent launch.
+// The entries of one launch: one element type, and one variant unless merged.
 struct BatchGroup {
   int dtype = 0;
-  uint32_t variant = 0;
+  uint32_t variant = 0;             // kernel key (batch_group_key)
   uint32_t nent = 0;
   uint32_t total_tiles = 0;
   uint32_t lds_bytes = 0;
@@ -100,7 +105,6 @@ struct BatchGroup {
 struct Batch {
   std::vector<BatchGroup> groups;
   uint32_t count = 0;
-  uint32_t *d_counters = nullptr;   // one work counter per group
   hipStream_t stream = nullptr;
   // Launches fork from the exec stream onto side streams and join back, so
   // the groups run concurrently (also inside the captured graph).
@@ -117,7 +121,7 @@ struct Batch {
 // Plan every item (tile shapes tuned per class when tuning_timeout != 0).
 int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, double tuning_timeout,
                 Batch &b, std::string &err);
-// Enqueue: counters reset + one persistent kernel per group.
+// Enqueue one persistent kernel per group, forked over the side streams.
 hipError_t batch_enqueue(Batch &b, hipStream_t stream);
 // Capture batch_enqueue into a graph (done on first exec).
 hipError_t batch_capture(Batch &b);

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -79,64 +80,66 @@ __device__ __forceinline__ void run_code(const BatchEntry &e, uint32_t lt, uint3
     tile_one<T, S, NC, vr, vw, upd, cc, sh, nt>(e.u.tile, lt, tid, A, B, smem, co);
 }
 
-// Persistent kernel for one variant: blocks take chunks of consecutive tiles
-// from `counter` (guided self-scheduling: chunks shrink as the queue drains)
-// until every tile is taken; every block leaves the loop once the counter
-// passes `total`.  One instance per (element type, variant), so each launch
-// carries only its variant's code and registers.
 template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
-__global__ __launch_bounds__(kBlock) void k_batch(const BatchEntry *__restrict__ ents,
-                                                  const uint32_t *__restrict__ begins,
-                                                  uint32_t nent, uint32_t total,
-                                                  uint32_t *counter) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ uint32_t s_range[2];
-  const uint32_t tid = threadIdx.x;
-  const uint32_t G = gridDim.x;
-  for (;;) {
-    if (tid == 0) {
-      const uint32_t cur = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t rem = cur < total ? total - cur : 0;
-      uint32_t chunk = rem / (4 * G);
-      chunk = chunk < 1 ? 1 : (chunk > 16 ? 16 : chunk);
-      const uint32_t old = atomicAdd(counter, chunk);
-      s_range[0] = old;
-      s_range[1] = old < total ? umin32(old + chunk, total) : old;
-    }
-    __syncthreads();
-    const uint32_t t0 = __builtin_amdgcn_readfirstlane(s_range[0]);
-    const uint32_t t1 = __builtin_amdgcn_readfirstlane(s_range[1]);
-    __syncthreads();
-    if (t0 >= total) break;
-    // Entry of t0: the last entry whose first tile is <= t0.  begins[0] = 0
-    // and begins is increasing, so lo ends at that entry.
-    uint32_t lo = 0, hi = nent;
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (__builtin_amdgcn_readfirstlane(begins[mid]) <= t0) lo = mid;
-      else hi = mid;
-    }
-    uint32_t ei = lo;
-    uint32_t end = (ei + 1 < nent) ? __builtin_amdgcn_readfirstlane(begins[ei + 1]) : total;
-    for (uint32_t t = t0; t < t1; ++t) {
-      while (t >= end && ei + 1 < nent) {
-        ++ei;
-        end = (ei + 1 < nent) ? __builtin_amdgcn_readfirstlane(begins[ei + 1]) : total;
+__device__ __forceinline__ void dispatch(uint32_t v, const BatchEntry &e, uint32_t lt,
+                                         uint32_t tid, unsigned char *smem) {
+  if constexpr (CODE < kVarCodes) {
+    if constexpr (valid_code<VCAP, CCOK>(CODE)) {
+      if (v == CODE) {
+        run_code<T, S, NC, VCAP, CCOK, CODE>(e, lt, tid, smem);
+        return;
       }
-      const uint32_t lt = t - __builtin_amdgcn_readfirstlane(begins[ei]);
-      run_code<T, S, NC, VCAP, CCOK, CODE>(ents[ei], lt, tid, smem);
     }
+    dispatch<T, S, NC, VCAP, CCOK, CODE + 1>(v, e, lt, tid, smem);
   }
 }
 
-__global__ void k_reset(uint32_t *c, uint32_t n) {
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) c[i] = 0;
+// Persistent kernel.  Tiles are taken in static round-robin order (block b
+// runs tiles b, b + G, b + 2G, ...), so at any moment the blocks work on
+// neighbouring tiles and sweep memory in order, as a one-tile-per-block
+// launch does, without a shared counter.  CODE >= 0: every entry has that
+// variant (one instance per variant); CODE == kMerged: each entry's variant
+// is dispatched at run time (one instance per element type).
+constexpr int kMerged = -1;
+
+template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
+__global__ __launch_bounds__(kBlock) void k_batch(const BatchEntry *__restrict__ ents,
+                                                  const uint32_t *__restrict__ begins,
+                                                  uint32_t nent, uint32_t total) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t tid = threadIdx.x;
+  uint32_t ei = 0;
+  uint32_t first = 0;
+  uint32_t end = (nent > 1) ? __builtin_amdgcn_readfirstlane(begins[1]) : total;
+  for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+    if (t >= end) {
+      // Entry of t: the last entry whose first tile is <= t, searched among
+      // the entries after the current one.
+      uint32_t lo = ei + 1, hi = nent;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (__builtin_amdgcn_readfirstlane(begins[mid]) <= t) lo = mid;
+        else hi = mid;
+      }
+      ei = lo;
+      first = __builtin_amdgcn_readfirstlane(begins[ei]);
+      end = (ei + 1 < nent) ? __builtin_amdgcn_readfirstlane(begins[ei + 1]) : total;
+    }
+    const BatchEntry &e = ents[ei];
+    if constexpr (CODE == kMerged)
+      dispatch<T, S, NC, VCAP, CCOK, 0>(__builtin_amdgcn_readfirstlane(e.variant), e, t - first,
+                                        tid, smem);
+    else
+      run_code<T, S, NC, VCAP, CCOK, CODE>(e, t - first, tid, smem);
+  }
 }
 
 // Host table of kernel instances: [dtype][variant code] (null where the code
 // does not exist for the type).
 template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
 void fill_kernels(const void **tab) {
+  if constexpr (CODE == 0)
+    tab[kVarCodes] = reinterpret_cast<const void *>(&k_batch<T, S, NC, VCAP, CCOK, kMerged>);
   if constexpr (CODE < kVarCodes) {
     if constexpr (valid_code<VCAP, CCOK>(CODE))
       tab[CODE] = reinterpret_cast<const void *>(&k_batch<T, S, NC, VCAP, CCOK, CODE>);
@@ -147,7 +150,7 @@ void fill_kernels(const void **tab) {
 }
 
 struct KernelTable {
-  const void *k[4][kVarCodes];
+  const void *k[4][kVarCodes + 1];   // [kVarCodes]: the merged instance
   KernelTable() {
     fill_kernels<Elem<float, 1>, float, 1, 4, false, 0>(k[kF32]);
     fill_kernels<Elem<double, 1>, double, 1, 2, false, 0>(k[kF64]);
@@ -158,7 +161,7 @@ struct KernelTable {
 
 const void *batch_kernel(int dtype, uint32_t variant) {
   static const KernelTable table;
-  if (dtype < kF32 || dtype > kC128 || variant >= (uint32_t)kVarCodes) return nullptr;
+  if (dtype < kF32 || dtype > kC128 || variant > (uint32_t)kVarCodes) return nullptr;
   return table.k[dtype][variant];
 }
 
@@ -279,18 +282,19 @@ uint32_t batch_grid(int dtype, uint32_t variant, uint32_t lds_bytes, uint32_t to
 
 hipError_t launch_batch_group(int dtype, uint32_t variant, const BatchEntry *d_ent,
                               const uint32_t *d_begins, uint32_t nent, uint32_t total,
-                              uint32_t lds_bytes, uint32_t grid, uint32_t *counter, hipStream_t st) {
+                              uint32_t lds_bytes, uint32_t grid, hipStream_t st) {
   if (!total || !nent) return hipSuccess;
   const void *k = batch_kernel(dtype, variant);
   if (!k) return hipErrorInvalidValue;
-  void *args[] = {(void *)&d_ent, (void *)&d_begins, (void *)&nent, (void *)&total, (void *)&counter};
+  void *args[] = {(void *)&d_ent, (void *)&d_begins, (void *)&nent, (void *)&total};
   return hipLaunchKernel(k, dim3(grid), dim3(kBlock), args, lds_bytes, st);
 }
 
-hipError_t batch_reset(uint32_t *counters, uint32_t n, hipStream_t st) {
-  if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, counters, n);
-  return hipGetLastError();
+uint32_t batch_group_key(uint32_t variant, bool merged) { return merged ? (uint32_t)kVarCodes : variant; }
+
+bool batch_merged_default() {
+  const char *m = std::getenv("HPTTX_BATCH_MODE");
+  return !(m && std::string(m) == "split");
 }
 
 int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, double tuning_timeout,
@@ -328,12 +332,13 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
       if (rc != kOk) { err = "item " + std::to_string(k) + ": " + err; return rc; }
     }
   }
+  const bool merged = batch_merged_default();
   std::map<std::pair<int, uint32_t>, std::vector<size_t>> by_kernel;
   for (size_t k = 0; k < items.size(); ++k) {
     b.paths[k] = pl[k].lp.path;
     BatchEntry probe;
     make_entry(pl[k], items[k], probe);
-    by_kernel[{items[k].dtype, probe.variant}].push_back(k);
+    by_kernel[{items[k].dtype, batch_group_key(probe.variant, merged)}].push_back(k);
   }
   hipError_t he = hipSuccess;
   for (auto &kv : by_kernel) {
@@ -374,8 +379,6 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
     b.groups.push_back(std::move(g));
     if (he != hipSuccess) break;
   }
-  if (he == hipSuccess) he = hipMalloc(&b.d_counters, sizeof(uint32_t) * std::max<size_t>(1, b.groups.size()));
-  if (he == hipSuccess) he = hipMemset(b.d_counters, 0, sizeof(uint32_t) * std::max<size_t>(1, b.groups.size()));
   // Side streams for concurrent groups; groups go, largest first, to the
   // least loaded lane.
   const size_t nside = std::min<size_t>(b.groups.size() > 0 ? b.groups.size() - 1 : 0, kBatchLanes - 1);
@@ -416,15 +419,15 @@ hipError_t batch_enqueue(Batch &b, hipStream_t st) {
   if (b.groups.empty()) return hipSuccess;
   // Counters are cleared by a kernel (not a memset node) so the sequence can
   // be captured in a graph and replayed.
-  hipError_t e = batch_reset(b.d_counters, (uint32_t)b.groups.size(), st);
-  if (e == hipSuccess && !b.side.empty()) e = hipEventRecord(b.events[0], st);
+  hipError_t e = hipSuccess;
+  if (!b.side.empty()) e = hipEventRecord(b.events[0], st);
   for (size_t i = 0; i < b.side.size() && e == hipSuccess; ++i)
     e = hipStreamWaitEvent(b.side[i], b.events[0], 0);
   for (size_t i = 0; i < b.groups.size() && e == hipSuccess; ++i) {
     const BatchGroup &g = b.groups[i];
     hipStream_t ls = b.lane[i] == 0 ? st : b.side[b.lane[i] - 1];
     e = launch_batch_group(g.dtype, g.variant, g.d_ent, g.d_begins, g.nent, g.total_tiles,
-                           g.lds_bytes, g.grid, b.d_counters + i, ls);
+                           g.lds_bytes, g.grid, ls);
   }
   for (size_t i = 0; i < b.side.size() && e == hipSuccess; ++i) {
     e = hipEventRecord(b.events[1 + i], b.side[i]);
@@ -463,8 +466,6 @@ void batch_free(Batch &b) {
     g.d_ent = nullptr;
     g.d_begins = nullptr;
   }
-  if (b.d_counters) (void)hipFree(b.d_counters);
-  b.d_counters = nullptr;
   for (hipStream_t s : b.side) (void)hipStreamDestroy(s);
   for (hipEvent_t e : b.events) (void)hipEventDestroy(e);
   b.side.clear();

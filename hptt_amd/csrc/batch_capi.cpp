@@ -149,10 +149,12 @@ int hpttx_batch_get_info(const hpttx_batch *h, hpttx_batch_info *info) {
   info->count = h->b.count;
   info->groups = (uint32_t)h->b.groups.size();
   info->algorithmic_bytes = h->b.bytes;
-  for (size_t i = 0; i < h->b.groups.size() && i < 4; ++i) {
+  for (size_t i = 0; i < h->b.groups.size(); ++i) {
     info->tiles += h->b.groups[i].total_tiles;
-    info->grid[i] = h->b.groups[i].grid;
-    info->lds_bytes[i] = h->b.groups[i].lds_bytes;
+    if (i < 4) {
+      info->grid[i] = h->b.groups[i].grid;
+      info->lds_bytes[i] = h->b.groups[i].lds_bytes;
+    }
   }
   return HPTTX_OK;
 }

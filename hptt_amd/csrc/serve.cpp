@@ -96,7 +96,6 @@ int Server::start(const ServerConfig &cfg, std::string &err) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&s.d_ent, sizeof(BatchEntry) * cfg_.max_batch);
     if (e == hipSuccess) e = hipMalloc(&s.d_begins, sizeof(uint32_t) * cfg_.max_batch);
-    if (e == hipSuccess) e = hipMalloc(&s.d_counters, sizeof(uint32_t) * kMaxSlotGroups);
     if (e == hipSuccess) e = hipHostMalloc(&s.h_ent, sizeof(BatchEntry) * cfg_.max_batch, hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc(&s.h_begins, sizeof(uint32_t) * cfg_.max_batch, hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -108,7 +107,7 @@ int Server::start(const ServerConfig &cfg, std::string &err) {
   // Look up the persistent kernels' occupancy for every LDS size a launch
   // can use now, not on the request path.
   for (int dt = kF32; dt <= kC128; ++dt)
-    for (uint32_t v = 0; v < 128; ++v)
+    for (uint32_t v = 0; v <= 128; ++v)
       for (uint32_t lds = 4096; lds <= 24576; lds += 4096) (void)batch_grid(dt, v, lds, 1u << 30);
   stop_ = false;
   dispatcher_ = std::thread([this] { dispatch_loop(); });
@@ -142,7 +141,6 @@ void Server::stop() {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     if (s.d_ent) (void)hipFree(s.d_ent);
     if (s.d_begins) (void)hipFree(s.d_begins);
-    if (s.d_counters) (void)hipFree(s.d_counters);
     if (s.h_ent) (void)hipHostFree(s.h_ent);
     if (s.h_begins) (void)hipHostFree(s.h_begins);
     if (s.done) (void)hipEventDestroy(s.done);
@@ -291,11 +289,13 @@ void Server::dispatch_loop() {
     stats_.launches++;
     lk.unlock();
 
-    // One launch per (element type, variant); within one, largest first.
+    // One launch per element type (or per (type, variant)); within one,
+    // largest first.
+    const bool merged = batch_merged_default();
     for (Request *r : batch) {
       BatchEntry probe;
       make_entry(r->plan, r->spec, probe);
-      r->variant = probe.variant;
+      r->variant = batch_group_key(probe.variant, merged);
     }
     std::stable_sort(batch.begin(), batch.end(), [](const Request *a, const Request *b) {
       if (a->spec.dtype != b->spec.dtype) return a->spec.dtype < b->spec.dtype;
@@ -332,12 +332,11 @@ void Server::dispatch_loop() {
       he = hipMemcpyAsync(slot->d_begins, slot->h_begins, sizeof(uint32_t) * n,
                           hipMemcpyHostToDevice, slot->stream);
     if (groups.size() > kMaxSlotGroups) he = hipErrorInvalidValue;
-    if (he == hipSuccess) he = batch_reset(slot->d_counters, (uint32_t)groups.size(), slot->stream);
     for (size_t gi = 0; gi < groups.size() && he == hipSuccess; ++gi) {
       const Group &g = groups[gi];
       he = launch_batch_group(g.dtype, g.variant, slot->d_ent + g.first, slot->d_begins + g.first,
                               g.count, g.total, g.lds, batch_grid(g.dtype, g.variant, g.lds, g.total),
-                              slot->d_counters + gi, slot->stream);
+                              slot->stream);
     }
     if (he == hipSuccess) he = hipEventRecord(slot->done, slot->stream);
     lk.lock();

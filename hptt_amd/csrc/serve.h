@@ -78,7 +78,6 @@ struct Slot {
   hipEvent_t done = nullptr;
   BatchEntry *d_ent = nullptr;
   uint32_t *d_begins = nullptr;
-  uint32_t *d_counters = nullptr;
   BatchEntry *h_ent = nullptr;     // pinned staging
   uint32_t *h_begins = nullptr;    // pinned staging
   std::vector<Request *> reqs;

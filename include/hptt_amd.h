@@ -191,10 +191,10 @@ int hpttx_batch_use_graph(hpttx_batch *batch, int enable);
 
 typedef struct hpttx_batch_info {
   uint32_t count;              /* items */
-  uint32_t groups;             /* element types present = launches per exec */
+  uint32_t groups;             /* (element type, variant) groups = launches per exec */
   uint64_t algorithmic_bytes;  /* sum over items of N*elem*(2 + update) */
   uint64_t tiles;              /* block tiles over all groups */
-  uint32_t grid[4];            /* persistent blocks per group */
+  uint32_t grid[4];            /* persistent blocks of the first four groups */
   uint32_t lds_bytes[4];
 } hpttx_batch_info;
 int hpttx_batch_get_info(const hpttx_batch *batch, hpttx_batch_info *info);

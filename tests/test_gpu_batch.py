@@ -72,7 +72,7 @@ def test_mixed_batch_vs_oracle(device):
     torch.cuda.synchronize()
     batch = _batch(items, bufs)
     info = batch.info()
-    assert info.count == 240 and info.groups == 4
+    assert info.count == 240 and info.groups >= 4   # one launch per (dtype, variant)
     batch.exec()
     batch.sync()
     for k, (it, b) in enumerate(zip(items, bufs)):
