@@ -82,10 +82,11 @@ hipError_t launch_batch_group(int dtype, uint32_t variant, const BatchEntry *d_e
 // Kernel key of an entry's group: its variant, or the per-type instance
 // that dispatches variants at run time when `merged`.
 uint32_t batch_group_key(uint32_t variant, bool merged);
-// Grouping used by batches and the server: one launch per element type
-// (merged, default) or one per (element type, variant) when the environment
-// variable HPTTX_BATCH_MODE is "split".
-bool batch_merged_default();
+// Grouping: one launch per element type (merged) or one per (element type,
+// variant) (split).  The environment variable HPTTX_BATCH_MODE ("merged" /
+// "split") overrides `fallback`; batches default to split, the server to
+// merged (fewer launches per slot: lower latency).
+bool batch_merged_default(bool fallback);
 
 // The entries of one launch: one element type, and one variant unless merged.
 struct BatchGroup {

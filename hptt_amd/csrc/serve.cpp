@@ -291,7 +291,7 @@ void Server::dispatch_loop() {
 
     // One launch per element type (or per (type, variant)); within one,
     // largest first.
-    const bool merged = batch_merged_default();
+    const bool merged = batch_merged_default(true);
     for (Request *r : batch) {
       BatchEntry probe;
       make_entry(r->plan, r->spec, probe);
