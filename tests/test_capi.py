@@ -69,3 +69,15 @@ def test_python_binding_loads_and_validates(lib):
         assert hptt_amd.last_error()
     # the drop-in returns None (nullptr) for the same cases
     assert hptt_amd.create_plan(fake, fake, [4], [0], 1.0, 0.0, dtype="s") is None
+
+
+def test_cpp_dropin_header_compiles(tmp_path):
+    """include/hptt/hptt.h compiles as plain C++17 against the C ABI header
+    (host compiler, no HIP headers needed)."""
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "hptt/hptt.h"\n'
+                   'int main() { auto p = hptt::create_plan<float>(nullptr, nullptr, {2, 2},'
+                   ' {1, 0}, 1.f, 0.f, 1); return p ? 1 : 0; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                        str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
