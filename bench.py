@@ -50,6 +50,35 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+DTYPE_TAG = {"float32": "f32", "float64": "f64", "complex64": "c64 (f32 pairs)",
+             "complex128": "c128 (f64 pairs)"}
+
+
+def describe_workload(name):
+    w = WORKLOADS[name]
+    fmt = lambda v: ("%g%+gi" % (v.real, v.imag)) if isinstance(v, complex) else "%g" % v
+    return "plain single transpose %s %d-D {%s} perm {%s} alpha=%s beta=%s" % (
+        w["dtype"], len(w["size"]), ",".join(map(str, w["size"])), ",".join(map(str, w["perm"])),
+        fmt(w["alpha"]), fmt(w["beta"]))
+
+
+TRAFFIC_FILE = os.path.join("profiles", "hbm_traffic_pmc.md")
+
+
+def pmc_traffic(name):
+    """HBM bytes per launch of the workload's kernel as measured by rocprofv3
+    FETCH_SIZE / WRITE_SIZE (scaled by a plain copy of known size), read from
+    the committed summary; None when it holds no figure for the workload."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
+            for ln in f:
+                if ln.startswith("traffic[%s] = " % name):
+                    return int(ln.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def time_workload(name, steps, warmup, tune=0.0):
     """Returns a dict with GB/s, ms per exec and plan info for one workload
     (tune > 0: autotune the plan for that many seconds first)."""
@@ -444,16 +473,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": DTYPE_TAG[WORKLOADS[args.workload]["dtype"]],
         "data": "synthetic: A, B uniform[-1,1) from a fixed seed, generated on the device",
-        "config": {"workload": "plain single transpose fp32 6-D {16,24,32,16,24,32} "
-                               "perm {5,2,0,4,1,3} alpha=2.3 beta=4.2",
+        "config": {"workload": describe_workload(args.workload),
                    "elements": main_res["elements"], "bytes_per_step": main_res["bytes_per_exec"],
                    "path": main_res["path"]},
         "pct_hbm_peak": main_res["pct_hbm_peak"],
         "reference_metric_gibps": main_res["reference_metric_gibps"],
         "roofline": {"bound": "hbm", "achieved": gbps, "peak": PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(gbps / PEAK_GBPS, 4), "traffic": None},
+                     "frac": round(gbps / PEAK_GBPS, 4), "traffic": pmc_traffic(args.workload),
+                     "traffic_source": TRAFFIC_FILE},
         "cpu_baseline": cpu,
         "secondary": secondary,
     }

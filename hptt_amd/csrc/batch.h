@@ -4,8 +4,10 @@
 // another: src/hptt.cc:16-93 + cgraph_trans.tcc:27-48).  Each transposition
 // is planned like a single plan (planner.h); its block tiles are numbered
 // consecutively, largest transposition first, and a persistent kernel per
-// element type takes chunks of tiles from an atomic work queue, finding each
-// tile's transposition by a search over the entries' first tiles.  The
+// launch group walks the tile numbers in a static round-robin order (block b
+// takes tiles b, b + grid, b + 2 grid, ...), finding each tile's
+// transposition by a binary search over the entries' first tiles.  The order
+// needs no work counter and keeps the blocks sweeping memory in step.  The
 // launches are captured in a hipGraph once and replayed.  The serving loop
 // (serve.h) drives the same kernel with per-launch tables.
 #pragma once

@@ -1,4 +1,5 @@
-// batch.hip -- persistent work-queue kernel over a batch of transpositions.
+// batch.hip -- persistent kernels over the tile queue of a batch of
+// transpositions (static round-robin order: block b takes tiles b, b + grid, ...).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -118,11 +118,12 @@ static bool divisible(int64_t x, uint32_t v) { return (x % (int64_t)v) == 0; }
 
 TuneParams default_tune(const Problem &p) {
   TuneParams t;
-  // Runs of 256 B on each side and ~8 KiB per block tile (exhaustive tuning
-  // of the fp32/fp64 6-D, complex<double> 4-D and copy workloads on MI355X
-  // picked 8 KiB tiles for every one of them).
-  t.run_r = std::max<uint32_t>(16, 256 / p.esize);
-  t.run_c = std::max<uint32_t>(16, 256 / p.esize);
+  // Read runs of 32 elements along A, write runs of 512 bytes along B (256
+  // for 16-byte elements) and ~8 KiB blocks: what exhaustive tuning of the
+  // 6-D fp32/fp64 and 4-D complex<double> workloads picked on MI355X
+  // (32x128, 32x64 and 32x16 tiles).
+  t.run_r = 32;
+  t.run_c = p.esize >= 16 ? 16 : 512 / p.esize;
   t.block_elems = 8192 / p.esize;
   t.wide_min_bytes = 256;
   return t;
