@@ -43,6 +43,11 @@ WORKLOADS = {
     # practical ceiling the transposes are compared with
     "copy-f32-512m": dict(size=[131072, 1024], perm=[0, 1], dtype="float32",
                           alpha=1.0, beta=0.0, flush=False),
+    # identity perm with beta on the main workload's element count: the same
+    # bytes and the same 2:1 read:write mix as the 6-D transposes, with no
+    # transposition -- their practical ceiling
+    "axpby-f32-151m": dict(size=[12288, 12288], perm=[0, 1], dtype="float32",
+                           alpha=2.3, beta=4.2, flush=False),
 }
 
 
@@ -437,7 +442,8 @@ def main():
                                                  main_res["pct_hbm_peak"]))
     secondary = {}
     if not args.quick:
-        for name in ("6d-f64", "4d-c128-tile", "4d-c128-wide", "3d-f32-copy", "copy-f32-512m"):
+        for name in ("6d-f64", "4d-c128-tile", "4d-c128-wide", "3d-f32-copy", "copy-f32-512m",
+                     "axpby-f32-151m"):
             if name == args.workload:
                 continue
             secondary[name] = time_workload(name, max(10, args.steps // 2), args.warmup, args.tune)

@@ -1,5 +1,7 @@
-// batch.hip -- persistent kernels over the tile queue of a batch of
-// transpositions (static round-robin order: block b takes tiles b, b + grid, ...).
+// batch.hip -- host side of batched transpositions: planning the items,
+// building the launch groups and their device tables, persistent grids,
+// launches over side streams and the captured hipGraph.  The kernels are in
+// batch_kernels.h (instantiated by batch_kernels_<type>.hip).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -12,6 +14,7 @@
 #include <tuple>
 
 #include "batch.h"
+#include "batch_kernels.h"
 #include "hpttx_core.h"
 #include "tune.h"
 
@@ -19,144 +22,13 @@ namespace hpttx {
 
 namespace {
 
-constexpr uint32_t kVarRows = 1, kVarVr = 2, kVarVw = 4, kVarUpd = 8, kVarSh = 16, kVarCc = 32,
-                   kVarNt = 64;
-constexpr int kVarCodes = 128;
-
-template <typename T, typename S, int NC, int VR, int VW, bool UPD, bool CC, bool SH, bool NT>
-__device__ __forceinline__ void tile_one(const TileDesc &d, uint32_t lt, uint32_t tid,
-                                         const T *A, T *B, unsigned char *smem,
-                                         const Coef<S> &co) {
-  T *lds = reinterpret_cast<T *>(smem);
-  const TileTabs tb = tile_tabs(d, smem, sizeof(T));
-  const TileCoord tc = tile_coord(d, lt);
-  tile_tables(d, tc, tid, tb);
-  __syncthreads();
-  WriteItems<T, VW, kUcapBatch> first;
-  tile_write_setup<T, VW, UPD, NT, kUcapBatch>(d, tc, tid, B, tb, first);
-  tile_read<T, VR, NT, kUcapBatch>(d, tc, tid, A, lds, tb);
-  __syncthreads();
-  tile_write<T, S, NC, VW, UPD, CC, SH, NT, kUcapBatch>(d, tc, tid, B, lds, tb, co, &first);
-  __syncthreads();
-}
-
-template <typename T, typename S, int NC, int V, bool UPD, bool CC, bool NT>
-__device__ __forceinline__ void rows_one(const RowDesc &d, uint32_t lt, uint32_t tid,
-                                         const T *A, T *B, unsigned char *smem,
-                                         const Coef<S> &co) {
-  int64_t *tabA = reinterpret_cast<int64_t *>(smem);
-  int64_t *tabB = tabA + d.RB;
-  const RowCoord rc = row_coord(d, lt);
-  row_tables(d, rc, tid, tabA, tabB);
-  __syncthreads();
-  row_body<T, S, NC, V, UPD, CC, NT, kUcapBatch>(d, rc, tid, A, B, tabA, tabB, co);
-  __syncthreads();
-}
-
-// Variant codes that exist for an element type: rows ignore the write-vector
-// and shared bits; scalar-only types (VCAP == 1) ignore the vector bits;
-// complex coefficients only for complex types.
-template <int VCAP, bool CCOK>
-constexpr bool valid_code(int c) {
-  return !((c & kVarRows) && (c & (kVarVw | kVarSh))) &&
-         !(VCAP == 1 && (c & (kVarVr | kVarVw))) && !(!CCOK && (c & kVarCc));
-}
-
-template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
-__device__ __forceinline__ void run_code(const BatchEntry &e, uint32_t lt, uint32_t tid,
-                                         unsigned char *smem) {
-  constexpr bool rows = CODE & kVarRows;
-  constexpr int vr = (CODE & kVarVr) ? VCAP : 1;
-  constexpr int vw = (CODE & kVarVw) ? VCAP : 1;
-  constexpr bool upd = CODE & kVarUpd;
-  constexpr bool sh = CODE & kVarSh;
-  constexpr bool cc = CODE & kVarCc;
-  constexpr bool nt = CODE & kVarNt;
-  const Coef<S> co{(S)e.ar, (S)e.ai, (S)e.br, (S)e.bi};
-  const T *A = static_cast<const T *>(e.A);
-  T *B = static_cast<T *>(e.B);
-  if constexpr (rows)
-    rows_one<T, S, NC, vr, upd, cc, nt>(e.u.rows, lt, tid, A, B, smem, co);
-  else
-    tile_one<T, S, NC, vr, vw, upd, cc, sh, nt>(e.u.tile, lt, tid, A, B, smem, co);
-}
-
-template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
-__device__ __forceinline__ void dispatch(uint32_t v, const BatchEntry &e, uint32_t lt,
-                                         uint32_t tid, unsigned char *smem) {
-  if constexpr (CODE < kVarCodes) {
-    if constexpr (valid_code<VCAP, CCOK>(CODE)) {
-      if (v == CODE) {
-        run_code<T, S, NC, VCAP, CCOK, CODE>(e, lt, tid, smem);
-        return;
-      }
-    }
-    dispatch<T, S, NC, VCAP, CCOK, CODE + 1>(v, e, lt, tid, smem);
-  }
-}
-
-// Persistent kernel.  Tiles are taken in static round-robin order (block b
-// runs tiles b, b + G, b + 2G, ...), so at any moment the blocks work on
-// neighbouring tiles and sweep memory in order, as a one-tile-per-block
-// launch does, without a shared counter.  CODE >= 0: every entry has that
-// variant (one instance per variant); CODE == kMerged: each entry's variant
-// is dispatched at run time (one instance per element type).
-constexpr int kMerged = -1;
-
-template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
-__global__ __launch_bounds__(kBlock) void k_batch(const BatchEntry *__restrict__ ents,
-                                                  const uint32_t *__restrict__ begins,
-                                                  uint32_t nent, uint32_t total) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const uint32_t tid = threadIdx.x;
-  uint32_t ei = 0;
-  uint32_t first = 0;
-  uint32_t end = (nent > 1) ? __builtin_amdgcn_readfirstlane(begins[1]) : total;
-  for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
-    if (t >= end) {
-      // Entry of t: the last entry whose first tile is <= t, searched among
-      // the entries after the current one.
-      uint32_t lo = ei + 1, hi = nent;
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (__builtin_amdgcn_readfirstlane(begins[mid]) <= t) lo = mid;
-        else hi = mid;
-      }
-      ei = lo;
-      first = __builtin_amdgcn_readfirstlane(begins[ei]);
-      end = (ei + 1 < nent) ? __builtin_amdgcn_readfirstlane(begins[ei + 1]) : total;
-    }
-    const BatchEntry &e = ents[ei];
-    if constexpr (CODE == kMerged)
-      dispatch<T, S, NC, VCAP, CCOK, 0>(__builtin_amdgcn_readfirstlane(e.variant), e, t - first,
-                                        tid, smem);
-    else
-      run_code<T, S, NC, VCAP, CCOK, CODE>(e, t - first, tid, smem);
-  }
-}
-
-// Host table of kernel instances: [dtype][variant code] (null where the code
-// does not exist for the type).
-template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
-void fill_kernels(const void **tab) {
-  if constexpr (CODE == 0)
-    tab[kVarCodes] = reinterpret_cast<const void *>(&k_batch<T, S, NC, VCAP, CCOK, kMerged>);
-  if constexpr (CODE < kVarCodes) {
-    if constexpr (valid_code<VCAP, CCOK>(CODE))
-      tab[CODE] = reinterpret_cast<const void *>(&k_batch<T, S, NC, VCAP, CCOK, CODE>);
-    else
-      tab[CODE] = nullptr;
-    fill_kernels<T, S, NC, VCAP, CCOK, CODE + 1>(tab);
-  }
-}
-
 struct KernelTable {
   const void *k[4][kVarCodes + 1];   // [kVarCodes]: the merged instance
   KernelTable() {
-    fill_kernels<Elem<float, 1>, float, 1, 4, false, 0>(k[kF32]);
-    fill_kernels<Elem<double, 1>, double, 1, 2, false, 0>(k[kF64]);
-    fill_kernels<Elem<float, 2>, float, 2, 2, true, 0>(k[kC64]);
-    fill_kernels<Elem<double, 2>, double, 2, 1, true, 0>(k[kC128]);
+    batch_kernels_f32(k[kF32]);
+    batch_kernels_f64(k[kF64]);
+    batch_kernels_c64(k[kC64]);
+    batch_kernels_c128(k[kC128]);
   }
 };
 
@@ -170,10 +42,10 @@ uint32_t entry_variant(const LaunchPlan &lp, bool update, bool cc, uint32_t vcap
   uint32_t v = 0;
   if (lp.kind == kRows) {
     v |= kVarRows;
-    if (lp.rows.v == vcap && vcap > 1) v |= kVarVr;
+    if (lp.rows.v == vcap && vcap > 1) v |= lp.rows.ua ? kVarVw : kVarVr;
   } else {
-    if (lp.tile.vr == vcap && vcap > 1) v |= kVarVr;
-    if (lp.tile.vw == vcap && vcap > 1) v |= kVarVw;
+    if (vcap > 1 && !lp.tile.ua_r) v |= kVarVr;
+    if (vcap > 1 && !lp.tile.ua_w) v |= kVarVw;
     if (lp.tile.VS > 1) v |= kVarSh;
   }
   if (update) v |= kVarUpd;
@@ -488,12 +360,14 @@ int batch_set_data(Batch &b, uint32_t k, const void *A, void *B, std::string &er
       if (g.order[i] != k) continue;
       BatchEntry &e = g.h_ent[i];
       const uint32_t es = dtype_esize(g.dtype);
-      // The entry's vector widths were chosen for the old pointers' 16-byte
-      // alignment; new pointers must allow the same widths.
+      // Exact-vector sides were chosen for the old pointers' 16-byte
+      // alignment; new pointers must keep it there (unaligned sides accept
+      // any element-aligned pointer).
       const bool vr = (e.variant & kVarVr) != 0, vw = (e.variant & kVarVw) != 0;
       const bool rows = (e.variant & kVarRows) != 0;
+      const bool need_a = vr, need_b = rows ? vr : vw;
       const uint32_t cap = 16 / es;
-      if ((vr && align_vec(A, es, cap) != cap) || ((vw || (rows && vr)) && align_vec(B, es, cap) != cap)) {
+      if ((need_a && align_vec(A, es, cap) != cap) || (need_b && align_vec(B, es, cap) != cap)) {
         err = "new pointers are not 16-byte aligned like the ones the batch was built for";
         return kEInval;
       }

@@ -1,0 +1,180 @@
+// batch_kernels.h -- the persistent batch kernel templates (device code).
+//
+// Included by one source file per element type (batch_kernels_<type>.hip),
+// each of which instantiates every variant of its type and fills a table of
+// kernel pointers, so the instances compile in parallel.  The host side of
+// the batch (planning, grids, launches) is batch.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "batch.h"
+#include "hpttx_core.h"
+
+namespace hpttx {
+
+// Variant bits of a batch entry (BatchEntry::variant).
+constexpr uint32_t kVarRows = 1, kVarVr = 2, kVarVw = 4, kVarUpd = 8, kVarSh = 16, kVarCc = 32,
+                   kVarNt = 64;
+constexpr int kVarCodes = 128;
+
+// Fill tab[0 .. kVarCodes] with the kernel instances of one element type:
+// tab[code] runs entries of that variant (null where the code does not exist
+// for the type), tab[kVarCodes] dispatches each entry's variant at run time.
+void batch_kernels_f32(const void **tab);
+void batch_kernels_f64(const void **tab);
+void batch_kernels_c64(const void **tab);
+void batch_kernels_c128(const void **tab);
+
+namespace bk {
+
+template <typename T, typename S, int NC, int VR, int VW, bool UR, bool UW, bool UPD, bool CC,
+          bool SH, bool NT, int UCAP>
+__device__ __forceinline__ void tile_one(const TileDesc &d, uint32_t lt, uint32_t tid,
+                                         const T *A, T *B, unsigned char *smem,
+                                         const Coef<S> &co) {
+  T *lds = reinterpret_cast<T *>(smem);
+  const TileTabs tb = tile_tabs(d, smem, sizeof(T));
+  const TileCoord tc = tile_coord(d, lt);
+  tile_tables(d, tc, tid, tb);
+  __syncthreads();
+  WriteItems<T, VW, UCAP> first;
+  tile_write_setup<T, VW, UPD, NT, UCAP, UW>(d, tc, tid, B, tb, first);
+  tile_read<T, VR, NT, UCAP, UR>(d, tc, tid, A, lds, tb);
+  __syncthreads();
+  tile_write<T, S, NC, VW, UPD, CC, SH, NT, UCAP, UW>(d, tc, tid, B, lds, tb, co, &first);
+  __syncthreads();
+}
+
+template <typename T, typename S, int NC, int V, bool UA, bool UPD, bool CC, bool NT, int UCAP>
+__device__ __forceinline__ void rows_one(const RowDesc &d, uint32_t lt, uint32_t tid,
+                                         const T *A, T *B, unsigned char *smem,
+                                         const Coef<S> &co) {
+  int64_t *tabA = reinterpret_cast<int64_t *>(smem);
+  int64_t *tabB = tabA + d.RB;
+  const RowCoord rc = row_coord(d, lt);
+  row_tables(d, rc, tid, tabA, tabB);
+  __syncthreads();
+  row_body<T, S, NC, V, UPD, CC, NT, UCAP, UA>(d, rc, tid, A, B, tabA, tabB, co);
+  __syncthreads();
+}
+
+// Variant codes that exist for an element type.  Tiles: kVarVr / kVarVw mark
+// exact read / write runs; without them the side uses unaligned full-width
+// vectors (scalars when VCAP == 1).  Rows: kVarVr = exact vectors, kVarVw =
+// unaligned vectors, neither = scalars (strided rows or VCAP == 1); rows
+// ignore the shared bit.  Scalar-only types (VCAP == 1) have no vector bits;
+// complex coefficients only for complex types.
+template <int VCAP, bool CCOK>
+constexpr bool valid_code(int c) {
+  return !((c & kVarRows) && (c & kVarSh)) &&
+         !((c & kVarRows) && (c & kVarVr) && (c & kVarVw)) &&
+         !(VCAP == 1 && (c & (kVarVr | kVarVw))) && !(!CCOK && (c & kVarCc));
+}
+
+// UCAP: accesses in flight per thread for narrow accesses (kUcapBatch: the
+// batch kernels keep 4, which measured faster on MI355X than 8 -- occupancy
+// counts for more than depth in flight across many small transpositions).
+template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE, int UCAP>
+__device__ __forceinline__ void run_code(const BatchEntry &e, uint32_t lt, uint32_t tid,
+                                         unsigned char *smem) {
+  constexpr bool rows = CODE & kVarRows;
+  constexpr bool exact_r = CODE & kVarVr;
+  constexpr bool exact_w = CODE & kVarVw;
+  constexpr bool upd = CODE & kVarUpd;
+  constexpr bool sh = CODE & kVarSh;
+  constexpr bool cc = CODE & kVarCc;
+  constexpr bool nt = CODE & kVarNt;
+  const Coef<S> co{(S)e.ar, (S)e.ai, (S)e.br, (S)e.bi};
+  const T *A = static_cast<const T *>(e.A);
+  T *B = static_cast<T *>(e.B);
+  if constexpr (rows) {
+    // exact_w on rows: unaligned vectors
+    constexpr int v = (exact_r || exact_w) ? VCAP : 1;
+    rows_one<T, S, NC, v, exact_w, upd, cc, nt, UCAP>(e.u.rows, lt, tid, A, B, smem, co);
+  } else {
+    constexpr bool ua = VCAP > 1;
+    tile_one<T, S, NC, VCAP, VCAP, ua && !exact_r, ua && !exact_w, upd, cc, sh, nt, UCAP>(
+        e.u.tile, lt, tid, A, B, smem, co);
+  }
+}
+
+template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
+__device__ __forceinline__ void dispatch(uint32_t v, const BatchEntry &e, uint32_t lt,
+                                         uint32_t tid, unsigned char *smem) {
+  if constexpr (CODE < kVarCodes) {
+    if constexpr (valid_code<VCAP, CCOK>(CODE)) {
+      if (v == CODE) {
+        run_code<T, S, NC, VCAP, CCOK, CODE, kUcapBatch>(e, lt, tid, smem);
+        return;
+      }
+    }
+    dispatch<T, S, NC, VCAP, CCOK, CODE + 1>(v, e, lt, tid, smem);
+  }
+}
+
+// Persistent kernel.  Tile slots are taken in static round-robin order
+// (block b runs slots b, b + G, b + 2G, ..., each renumbered to a tile so
+// that one XCD runs neighbouring tiles), so at any moment the blocks work on
+// neighbouring tiles and sweep memory in order, as a one-tile-per-block
+// launch does, without a shared counter.  CODE >= 0: every entry has that
+// variant (one instance per variant); CODE == kMerged: each entry's variant
+// is dispatched at run time (one instance per element type).
+constexpr int kMerged = -1;
+
+template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
+__global__ __launch_bounds__(kBlock) void k_batch(const BatchEntry *__restrict__ ents,
+                                                  const uint32_t *__restrict__ begins,
+                                                  uint32_t nent, uint32_t total) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t full64 = total & ~63u;
+  uint32_t ei = 0;
+  uint32_t first = 0;
+  uint32_t end = (nent > 1) ? __builtin_amdgcn_readfirstlane(begins[1]) : total;
+  for (uint32_t i = blockIdx.x; i < total; i += gridDim.x) {
+    // Blocks are dispatched round-robin over the 8 XCDs, and with a grid
+    // that is a multiple of 8 the tiles i with i % 8 == x all run on XCD x.
+    // Renumber each group of 64 so that those are 8 consecutive tiles:
+    // neighbouring tiles share cache lines at their edges, and each XCD has
+    // its own L2.  (A bijection on [0, total); the last partial group keeps
+    // its order.)
+    const uint32_t t = i < full64 ? (i & ~63u) | ((i & 7u) << 3) | ((i >> 3) & 7u) : i;
+    if (t < first || t >= end) {
+      // Entry of t: the last entry whose first tile is <= t.
+      uint32_t lo = 0, hi = nent;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (__builtin_amdgcn_readfirstlane(begins[mid]) <= t) lo = mid;
+        else hi = mid;
+      }
+      ei = lo;
+      first = __builtin_amdgcn_readfirstlane(begins[ei]);
+      end = (ei + 1 < nent) ? __builtin_amdgcn_readfirstlane(begins[ei + 1]) : total;
+    }
+    const BatchEntry &e = ents[ei];
+    if constexpr (CODE == kMerged)
+      dispatch<T, S, NC, VCAP, CCOK, 0>(__builtin_amdgcn_readfirstlane(e.variant), e, t - first,
+                                        tid, smem);
+    else
+      run_code<T, S, NC, VCAP, CCOK, CODE, kUcapBatch>(e, t - first, tid, smem);
+  }
+}
+
+// Host table of kernel instances: [dtype][variant code] (null where the code
+// does not exist for the type).
+template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
+void fill_kernels(const void **tab) {
+  if constexpr (CODE == 0)
+    tab[kVarCodes] = reinterpret_cast<const void *>(&k_batch<T, S, NC, VCAP, CCOK, kMerged>);
+  if constexpr (CODE < kVarCodes) {
+    if constexpr (valid_code<VCAP, CCOK>(CODE))
+      tab[CODE] = reinterpret_cast<const void *>(&k_batch<T, S, NC, VCAP, CCOK, CODE>);
+    else
+      tab[CODE] = nullptr;
+    fill_kernels<T, S, NC, VCAP, CCOK, CODE + 1>(tab);
+  }
+}
+
+}  // namespace bk
+}  // namespace hpttx

@@ -43,8 +43,8 @@ namespace hpttx {
 constexpr int kMaxRank = 6;
 constexpr uint32_t kBlock = 256;   // threads per block (4 wave64)
 // Cap on accesses in flight per thread for narrow (sub-16-byte) accesses:
-// single-plan kernels afford 8; the batch kernel, which holds every variant's
-// registers at once, keeps 4.
+// single-plan kernels afford 8; the batch kernels keep 4 (higher occupancy
+// measured faster over many small transpositions).
 constexpr int kUcapPlan = 8;
 constexpr int kUcapBatch = 4;
 // Streaming mode (a per-plan choice): loads and stores of A and B carry the
@@ -123,9 +123,13 @@ struct TileDesc {
   uint32_t nR, nC, nO;
   uint32_t ntiles;
   uint32_t lds_row;      // LDS row stride in elements (multiple of the read vector)
-  uint32_t nvr;          // vectors per read run  (TR*VS / vr)
-  uint32_t nvw;          // vectors per write run (TC*VS / vw)
+  uint32_t nvr;          // vectors per read run  (ceil(TR*VS / vr))
+  uint32_t nvw;          // vectors per write run (ceil(TC*VS / vw))
   uint32_t vr, vw;       // vector widths in elements
+  // Run mode per side: 0 = exact (every run starts 16-byte aligned and is a
+  // whole number of vectors), 1 = unaligned (vectors at any element offset,
+  // a run's last partial vector done element by element).
+  uint32_t ua_r, ua_w;
   FastDiv div_nR, div_nC, div_nvr, div_TC, div_nvw, div_TR, div_VS;
   DimList R, C, O;
 };
@@ -136,7 +140,7 @@ struct TileDesc {
 // B order.  A block tile is one segment of SEG elements of RB rows.
 struct RowDesc {
   uint32_t L, SEG, nseg, nrows, RB, nrb, ntiles, v, nvs;
-  uint32_t pad_;
+  uint32_t ua;           // 1: vectors at any element offset, partial last vector
   int64_t ra, rb;
   FastDiv div_nseg, div_nvs;
   DimList W;
@@ -156,15 +160,24 @@ struct alignas(sizeof(S) * NC) Elem {
 
 typedef uint32_t u32x2 __attribute__((vector_size(8)));
 typedef uint32_t u32x4 __attribute__((vector_size(16)));
+// The same vectors at dword alignment: gfx950 global loads and stores of two
+// or four dwords need only a dword-aligned address (the backend emits one
+// global_load/store_dwordx2/x4 for them); LDS vector accesses stay aligned.
+typedef uint32_t u32x2u __attribute__((vector_size(8), aligned(4)));
+typedef uint32_t u32x4u __attribute__((vector_size(16), aligned(4)));
 
-template <int BYTES> struct RawVec;
-template <> struct RawVec<4> { typedef uint32_t type; };
-template <> struct RawVec<8> { typedef u32x2 type; };
-template <> struct RawVec<16> { typedef u32x4 type; };
+template <int BYTES, bool UA> struct RawVec;
+template <> struct RawVec<4, false> { typedef uint32_t type; };
+template <> struct RawVec<8, false> { typedef u32x2 type; };
+template <> struct RawVec<16, false> { typedef u32x4 type; };
+template <> struct RawVec<4, true> { typedef uint32_t type; };
+template <> struct RawVec<8, true> { typedef u32x2u type; };
+template <> struct RawVec<16, true> { typedef u32x4u type; };
 
-template <typename T, int V, bool NT = false>
+// UA: the address is only element-aligned (global memory only).
+template <typename T, int V, bool NT = false, bool UA = false>
 HPTTX_HD void vload(const T *p, T (&out)[V]) {
-  typedef typename RawVec<sizeof(T) * V>::type R;
+  typedef typename RawVec<sizeof(T) * V, UA>::type R;
   HPTTX_ACCESS(p, sizeof(R), false);
 #if defined(__HIP_DEVICE_COMPILE__)
   R r = NT ? __builtin_nontemporal_load(reinterpret_cast<const R *>(p))
@@ -175,9 +188,9 @@ HPTTX_HD void vload(const T *p, T (&out)[V]) {
   __builtin_memcpy(&out[0], &r, sizeof(R));
 }
 
-template <typename T, int V, bool NT>
+template <typename T, int V, bool NT, bool UA = false>
 HPTTX_HD void vstore(T *p, const T (&in)[V]) {
-  typedef typename RawVec<sizeof(T) * V>::type R;
+  typedef typename RawVec<sizeof(T) * V, UA>::type R;
   R r;
   HPTTX_ACCESS(p, sizeof(R), true);
   __builtin_memcpy(&r, &in[0], sizeof(R));
@@ -311,9 +324,35 @@ constexpr int unroll_for() {
        : (64 / (int)(V * sizeof(T))) > CAP ? CAP : (64 / (int)(V * sizeof(T)));
 }
 
+// Load the n <= V elements of a partial vector one by one (the rest of `out`
+// is left zero and never stored to B).
+template <typename T, int V, bool NT>
+HPTTX_HD void load_part(const T *p, uint32_t n, T (&out)[V]) {
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    T one[1];
+    __builtin_memset(&one[0], 0, sizeof(T));
+    if ((uint32_t)j < n) vload<T, 1, NT>(p + j, one);
+    out[j] = one[0];
+  }
+}
+
+template <typename T, int V, bool NT>
+HPTTX_HD void store_part(T *p, uint32_t n, const T (&in)[V]) {
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    if ((uint32_t)j < n) {
+      const T one[1] = {in[j]};
+      vstore<T, 1, NT>(p + j, one);
+    }
+  }
+}
+
 // Read pass: item k = (row, v), row = o*TC + c; one vector of VR elements
-// from A's run r0*VS + [0, rlen*VS) into LDS row `row`.
-template <typename T, int VR, bool NT, int UCAP>
+// from A's run r0*VS + [0, rlen*VS) into LDS row `row`.  UA: the run may
+// start at any element offset and end in a partial vector (loaded element by
+// element); the LDS side stays vector-aligned.
+template <typename T, int VR, bool NT, int UCAP, bool UA = false>
 HPTTX_HD void tile_read(const TileDesc &d, const TileCoord &tc, uint32_t tid,
                         const T *HPTTX_RESTRICT A, T *lds, const TileTabs &tb) {
   constexpr int U = unroll_for<T, VR, UCAP>();
@@ -334,7 +373,13 @@ HPTTX_HD void tile_read(const TileDesc &d, const TileCoord &tc, uint32_t tid,
       const uint32_t c = row - o * d.TC;
       ok[u] = (k < items) && (c < tc.clen) && (v * VR < run);
       lidx[u] = row * d.lds_row + v * VR;
-      if (ok[u]) vload<T, VR, NT>(A + abase + v * VR + tb.AC[c] + tb.AO[o], buf[u]);
+      if (ok[u]) {
+        const T *src = A + abase + v * VR + tb.AC[c] + tb.AO[o];
+        if (!UA || v * VR + VR <= run)
+          vload<T, VR, NT, UA>(src, buf[u]);
+        else
+          load_part<T, VR, NT>(src, run - v * VR, buf[u]);
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
@@ -348,6 +393,8 @@ HPTTX_HD void tile_read(const TileDesc &d, const TileCoord &tc, uint32_t tid,
 // be set up -- and, when B is read, B's values loaded -- before the read pass,
 // overlapping the B-read latency with the A-read latency.  Only the item
 // index and B's values are kept between the halves; addresses are recomputed.
+// UA: runs start at any element offset and may end in a partial vector,
+// which is read, gathered and written element by element.
 template <typename T, int VW, int UCAP>
 struct WriteItems {
   static constexpr int U = unroll_for<T, VW, UCAP>();
@@ -359,6 +406,7 @@ struct WriteAddr {
   int64_t boff;     // B element offset of the vector
   uint32_t lbase;   // LDS offset of (o, c = 0, r, s = 0)
   uint32_t q0;      // first element of the vector within the run
+  uint32_t n;       // elements of the vector inside the run (vw unless partial)
   bool ok;
 };
 
@@ -370,15 +418,17 @@ HPTTX_HD WriteAddr write_addr(const TileDesc &d, const TileCoord &tc, uint32_t k
   const uint32_t v = kk - row * nvw;
   const uint32_t o = fdiv(row, d.div_TR);
   const uint32_t r = row - o * d.TR;
+  const uint32_t run = tc.clen * d.VS;
   WriteAddr w;
-  w.ok = (kk < items) && (r < tc.rlen) && (v * vw < tc.clen * d.VS);
+  w.ok = (kk < items) && (r < tc.rlen) && (v * vw < run);
   w.q0 = v * vw;
+  w.n = w.ok ? umin32(vw, run - w.q0) : 0;
   w.lbase = o * d.TC * d.lds_row + r * d.VS;
   w.boff = w.ok ? (int64_t)tc.c0 * d.VS + v * vw + tb.BR[r] + tb.BO[o] : 0;
   return w;
 }
 
-template <typename T, int VW, bool UPD, bool NT, int UCAP>
+template <typename T, int VW, bool UPD, bool NT, int UCAP, bool UA = false>
 HPTTX_HD void tile_write_setup(const TileDesc &d, const TileCoord &tc, uint32_t k0,
                                const T *HPTTX_RESTRICT B, const TileTabs &tb,
                                WriteItems<T, VW, UCAP> &w) {
@@ -388,11 +438,17 @@ HPTTX_HD void tile_write_setup(const TileDesc &d, const TileCoord &tc, uint32_t 
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const WriteAddr a = write_addr(d, tc, k0 + (uint32_t)u * kBlock, VW, tb);
-    if (a.ok) vload<T, VW, NT>(B + a.boff, w.y[u]);
+    if (a.ok) {
+      if (!UA || a.n == VW)
+        vload<T, VW, NT, UA>(B + a.boff, w.y[u]);
+      else
+        load_part<T, VW, NT>(B + a.boff, a.n, w.y[u]);
+    }
   }
 }
 
-template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT, int UCAP>
+template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT, int UCAP,
+          bool UA = false>
 HPTTX_HD void tile_write_finish(const TileDesc &d, const TileCoord &tc, T *HPTTX_RESTRICT B,
                                 const T *lds, const TileTabs &tb,
                                 const WriteItems<T, VW, UCAP> &w, const Coef<S> &k) {
@@ -404,7 +460,9 @@ HPTTX_HD void tile_write_finish(const TileDesc &d, const TileCoord &tc, T *HPTTX
     T x[VW];
 #pragma unroll
     for (int j = 0; j < VW; ++j) {
-      const uint32_t q = a.q0 + j;
+      // elements past a partial vector's end read element 0 of the run
+      // (in the tile, never stored)
+      const uint32_t q = (!UA || (uint32_t)j < a.n) ? a.q0 + j : 0;
       uint32_t c = q, s = 0;
       if (SH) {
         c = fdiv(q, d.div_VS);
@@ -422,13 +480,17 @@ HPTTX_HD void tile_write_finish(const TileDesc &d, const TileCoord &tc, T *HPTTX
       Elem<S, NC> re = axpby<S, NC, UPD, CC>(xe, ye, k);
       __builtin_memcpy(&out[j], &re, sizeof(T));
     }
-    vstore<T, VW, NT>(B + a.boff, out);
+    if (!UA || a.n == VW)
+      vstore<T, VW, NT, UA>(B + a.boff, out);
+    else
+      store_part<T, VW, NT>(B + a.boff, a.n, out);
   }
 }
 
 // The whole write pass; `first` holds the thread's first items already set up
 // by tile_write_setup(k0 = tid) (with B prefetched), or is null.
-template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT, int UCAP>
+template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT, int UCAP,
+          bool UA = false>
 HPTTX_HD void tile_write(const TileDesc &d, const TileCoord &tc, uint32_t tid,
                          T *HPTTX_RESTRICT B, const T *lds, const TileTabs &tb,
                          const Coef<S> &k, const WriteItems<T, VW, UCAP> *first) {
@@ -436,13 +498,13 @@ HPTTX_HD void tile_write(const TileDesc &d, const TileCoord &tc, uint32_t tid,
   const uint32_t items = d.nvw * d.TR * tc.olen;
   uint32_t k0 = tid;
   if (first) {
-    tile_write_finish<T, S, NC, VW, UPD, CC, SH, NT, UCAP>(d, tc, B, lds, tb, *first, k);
+    tile_write_finish<T, S, NC, VW, UPD, CC, SH, NT, UCAP, UA>(d, tc, B, lds, tb, *first, k);
     k0 += kBlock * U;
   }
   for (; k0 < items; k0 += kBlock * U) {
     WriteItems<T, VW, UCAP> w;
-    tile_write_setup<T, VW, UPD, NT, UCAP>(d, tc, k0, B, tb, w);
-    tile_write_finish<T, S, NC, VW, UPD, CC, SH, NT, UCAP>(d, tc, B, lds, tb, w, k);
+    tile_write_setup<T, VW, UPD, NT, UCAP, UA>(d, tc, k0, B, tb, w);
+    tile_write_finish<T, S, NC, VW, UPD, CC, SH, NT, UCAP, UA>(d, tc, B, lds, tb, w, k);
   }
 }
 
@@ -475,7 +537,10 @@ HPTTX_HD void row_tables(const RowDesc &d, const RowCoord &rc, uint32_t tid,
   }
 }
 
-template <typename T, typename S, int NC, int V, bool UPD, bool CC, bool NT, int UCAP>
+// UA: rows start at any element offset in A and B and the segment may end
+// in a partial vector (done element by element).
+template <typename T, typename S, int NC, int V, bool UPD, bool CC, bool NT, int UCAP,
+          bool UA = false>
 HPTTX_HD void row_body(const RowDesc &d, const RowCoord &rc, uint32_t tid,
                        const T *HPTTX_RESTRICT A, T *HPTTX_RESTRICT B,
                        const int64_t *tabA, const int64_t *tabB, const Coef<S> &k) {
@@ -485,22 +550,30 @@ HPTTX_HD void row_body(const RowDesc &d, const RowCoord &rc, uint32_t tid,
   for (uint32_t k0 = tid; k0 < items; k0 += kBlock * U) {
     T x[U][V];
     T y[U][V];
-    bool ok[U];
+    uint32_t n[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t kk = k0 + (uint32_t)u * kBlock;
       const uint32_t rr = fdiv(kk, d.div_nvs);
       const uint32_t v = kk - rr * nvs;
-      ok[u] = (kk < items) && (v * V < rc.slen);
-      if (ok[u]) {
+      const bool ok = (kk < items) && (v * V < rc.slen);
+      n[u] = ok ? umin32(V, rc.slen - v * V) : 0;
+      if (ok) {
         const int64_t e = (int64_t)rc.seg0 + (int64_t)v * V;
-        vload<T, V, NT>(A + tabA[rr] + e * d.ra, x[u]);
-        if (UPD) vload<T, V, NT>(B + tabB[rr] + e * d.rb, y[u]);
+        const T *pa = A + tabA[rr] + e * d.ra;
+        const T *pb = B + tabB[rr] + e * d.rb;
+        if (!UA || n[u] == V) {
+          vload<T, V, NT, UA>(pa, x[u]);
+          if (UPD) vload<T, V, NT, UA>(pb, y[u]);
+        } else {
+          load_part<T, V, NT>(pa, n[u], x[u]);
+          if (UPD) load_part<T, V, NT>(pb, n[u], y[u]);
+        }
       }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (!ok[u]) continue;
+      if (!n[u]) continue;
       const uint32_t kk = k0 + (uint32_t)u * kBlock;
       const uint32_t rr = fdiv(kk, d.div_nvs);
       const uint32_t v = kk - rr * nvs;
@@ -513,7 +586,11 @@ HPTTX_HD void row_body(const RowDesc &d, const RowCoord &rc, uint32_t tid,
         Elem<S, NC> re = axpby<S, NC, UPD, CC>(xe, ye, k);
         __builtin_memcpy(&out[j], &re, sizeof(T));
       }
-      vstore<T, V, NT>(B + tabB[rr] + e * d.rb, out);
+      T *pb = B + tabB[rr] + e * d.rb;
+      if (!UA || n[u] == V)
+        vstore<T, V, NT, UA>(pb, out);
+      else
+        store_part<T, V, NT>(pb, n[u], out);
     }
   }
 }

@@ -11,9 +11,11 @@
 //       (reference MacroTransLinear, macro_kernel_trans.cc:148-195,
 //       kernel_trans_avx2.cc:480-530), the identity/axpby case and a strided
 //       gather for layouts without a unit-stride dim.
-// Loads and stores carry the non-temporal hint (the reference streams its
-// stores only when B is not read, macro_kernel_trans.cc:60-62; on gfx950 the
-// hint pays for loads and stores alike).
+// In streaming mode (chosen per plan by the planner or the tuner) loads and
+// stores carry the non-temporal hint.  The reference streams its stores only
+// when B is not read (macro_kernel_trans.cc:60-62); on gfx950 the hint pays
+// for loads and stores alike once every access is part of a long run and the
+// problem is larger than the last-level cache.
 #include <hip/hip_runtime.h>
 
 #include "hpttx_core.h"
@@ -22,8 +24,11 @@
 namespace hpttx {
 
 // One block per tile on a 2-D grid (t = y * gridDim.x + x).  The write pass's
-// first items are set up, and B prefetched, before the read pass.
-template <typename T, typename S, int NC, int VR, int VW, bool UPD, bool CC, bool SH, bool NT>
+// first items are set up, and B prefetched, before the read pass.  UR / UW:
+// unaligned runs on the read / write side (full-width vectors at any element
+// offset, partial last vectors element by element).
+template <typename T, typename S, int NC, int VR, int VW, bool UR, bool UW, bool UPD, bool CC,
+          bool SH, bool NT>
 __global__ __launch_bounds__(kBlock) void k_tile(TileDesc d, const T *__restrict__ A,
                                                  T *__restrict__ B, Coef<S> co) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -36,13 +41,13 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileDesc d, const T *__restrict
   tile_tables(d, tc, tid, tb);
   __syncthreads();
   WriteItems<T, VW, kUcapPlan> first;
-  tile_write_setup<T, VW, UPD, NT, kUcapPlan>(d, tc, tid, B, tb, first);
-  tile_read<T, VR, NT, kUcapPlan>(d, tc, tid, A, lds, tb);
+  tile_write_setup<T, VW, UPD, NT, kUcapPlan, UW>(d, tc, tid, B, tb, first);
+  tile_read<T, VR, NT, kUcapPlan, UR>(d, tc, tid, A, lds, tb);
   __syncthreads();
-  tile_write<T, S, NC, VW, UPD, CC, SH, NT, kUcapPlan>(d, tc, tid, B, lds, tb, co, &first);
+  tile_write<T, S, NC, VW, UPD, CC, SH, NT, kUcapPlan, UW>(d, tc, tid, B, lds, tb, co, &first);
 }
 
-template <typename T, typename S, int NC, int V, bool UPD, bool CC, bool NT>
+template <typename T, typename S, int NC, int V, bool UA, bool UPD, bool CC, bool NT>
 __global__ __launch_bounds__(kBlock) void k_rows(RowDesc d, const T *__restrict__ A,
                                                  T *__restrict__ B, Coef<S> co) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -54,7 +59,7 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowDesc d, const T *__restrict_
   const RowCoord rc = row_coord(d, t);
   row_tables(d, rc, tid, tabA, tabB);
   __syncthreads();
-  row_body<T, S, NC, V, UPD, CC, NT, kUcapPlan>(d, rc, tid, A, B, tabA, tabB, co);
+  row_body<T, S, NC, V, UPD, CC, NT, kUcapPlan, UA>(d, rc, tid, A, B, tabA, tabB, co);
 }
 
 namespace {
@@ -68,34 +73,38 @@ hipError_t launch_typed(const LaunchPlan &lp, const void *A, void *B,
   // 2-D grid: x * 256 threads must stay below 2^32 work items per dimension.
   const uint32_t gx = lp.grid < kGridX ? lp.grid : kGridX;
   dim3 grid(gx, (lp.grid + gx - 1) / gx), block(kBlock);
+  // Per side: exact vectors, unaligned vectors (VCAP > 1) or scalars (VCAP == 1).
+  constexpr bool kUA = VCAP > 1;
   if (lp.kind == kTile) {
     const TileDesc &d = lp.tile;
     const bool sh = d.VS > 1;
-    const bool vr = d.vr == VCAP && VCAP > 1;
-    const bool vw = d.vw == VCAP && VCAP > 1;
-#define HPTTX_TILE(VR_, VW_, SH_)                                                    \
-  hipLaunchKernelGGL((k_tile<T, S, NC, VR_, VW_, UPD, CC, SH_, NT>), grid, block,  \
+    const bool ur = d.ua_r != 0, uw = d.ua_w != 0;
+#define HPTTX_TILE(UR_, UW_, SH_)                                                              \
+  hipLaunchKernelGGL((k_tile<T, S, NC, VCAP, VCAP, UR_, UW_, UPD, CC, SH_, NT>), grid, block, \
                      lp.lds_bytes, st, d, a, b, co)
     if (sh) {
-      if (vr && vw) HPTTX_TILE(VCAP, VCAP, true);
-      else if (vr) HPTTX_TILE(VCAP, 1, true);
-      else if (vw) HPTTX_TILE(1, VCAP, true);
-      else HPTTX_TILE(1, 1, true);
+      if (!ur && !uw) HPTTX_TILE(false, false, true);
+      else if (!ur) HPTTX_TILE(false, kUA, true);
+      else if (!uw) HPTTX_TILE(kUA, false, true);
+      else HPTTX_TILE(kUA, kUA, true);
     } else {
-      if (vr && vw) HPTTX_TILE(VCAP, VCAP, false);
-      else if (vr) HPTTX_TILE(VCAP, 1, false);
-      else if (vw) HPTTX_TILE(1, VCAP, false);
-      else HPTTX_TILE(1, 1, false);
+      if (!ur && !uw) HPTTX_TILE(false, false, false);
+      else if (!ur) HPTTX_TILE(false, kUA, false);
+      else if (!uw) HPTTX_TILE(kUA, false, false);
+      else HPTTX_TILE(kUA, kUA, false);
     }
 #undef HPTTX_TILE
   } else {
     const RowDesc &d = lp.rows;
-    if (d.v == VCAP && VCAP > 1)
-      hipLaunchKernelGGL((k_rows<T, S, NC, VCAP, UPD, CC, NT>), grid, block, lp.lds_bytes,
-                         st, d, a, b, co);
+    if (d.v == VCAP && VCAP > 1 && !d.ua)
+      hipLaunchKernelGGL((k_rows<T, S, NC, VCAP, false, UPD, CC, NT>), grid, block,
+                         lp.lds_bytes, st, d, a, b, co);
+    else if (d.v == VCAP && VCAP > 1)
+      hipLaunchKernelGGL((k_rows<T, S, NC, VCAP, kUA, UPD, CC, NT>), grid, block,
+                         lp.lds_bytes, st, d, a, b, co);
     else
-      hipLaunchKernelGGL((k_rows<T, S, NC, 1, UPD, CC, NT>), grid, block, lp.lds_bytes, st,
-                         d, a, b, co);
+      hipLaunchKernelGGL((k_rows<T, S, NC, 1, false, UPD, CC, NT>), grid, block, lp.lds_bytes,
+                         st, d, a, b, co);
   }
   return hipGetLastError();
 }

@@ -183,7 +183,7 @@ static int plan_rows(const Problem &p, int rowdim, const TuneParams &tp,
   // A block tile covers ~block_elems elements: one segment of RB rows.
   const uint32_t be = std::max<uint32_t>(tp.block_elems, kBlock);
   uint32_t seg = (uint32_t)std::min<uint64_t>(L, be);
-  if (seg < L) seg = std::max<uint32_t>(v, seg - seg % v);
+  if (seg < L) seg = std::max<uint32_t>(v, seg - seg % vcap);
   uint32_t rb = std::max<uint32_t>(1, be / std::max<uint32_t>(seg, 1));
   rb = std::min<uint32_t>(rb, 1024);
   rb = (uint32_t)std::min<uint64_t>(rb, nrows);
@@ -307,8 +307,7 @@ static int plan_tile(const Problem &p, bool shared, const TuneParams &tp,
   // then, while the tile holds less than half of block_elems, lengthen the
   // shorter run (doubling) as far as its super-dim allows; the O count fills
   // the rest.  Each extent is then balanced over its tile count (no sliver
-  // edge tiles) and kept a multiple of the vector width where the shape
-  // allows full vectors.
+  // edge tiles).
   auto cdiv = [](uint64_t a, uint64_t b) { return (a + b - 1) / b; };
   uint64_t TR = std::min<uint64_t>(VR, std::max<uint64_t>(1, cdiv(tp.run_r, VS)));
   uint64_t TC = std::min<uint64_t>(VC, std::max<uint64_t>(1, cdiv(tp.run_c, VS)));
@@ -320,17 +319,19 @@ static int plan_tile(const Problem &p, bool shared, const TuneParams &tp,
     if (grow_r) TR = std::min<uint64_t>(VR, TR * 2);
     else TC = std::min<uint64_t>(VC, TC * 2);
   }
+  // Extents are rounded to whole 16-byte vectors (interior tiles then need
+  // no partial vector even where the runs are unaligned).
   auto balance = [&](uint64_t V, uint64_t T, uint32_t vec) {
     const uint64_t n = cdiv(V, T);
     uint64_t b = cdiv(V, n);
     // vector granularity in index units: runs are index*VS elements
     uint64_t g = 1;
     while (g < vec && ((g * VS) % vec) != 0) ++g;
-    if ((VS * V) % vec == 0 && g > 1) b = std::min<uint64_t>(V, cdiv(b, g) * g);
+    if (g > 1) b = std::min<uint64_t>(V, cdiv(b, g) * g);
     return std::max<uint64_t>(1, b);
   };
-  TR = balance(VR, TR, vr);
-  TC = balance(VC, TC, vw);
+  TR = balance(VR, TR, vcap);
+  TC = balance(VC, TC, vcap);
   // Whole-tile runs must stay multiples of the vector widths.
   while (vr > 1 && (TR * VS) % vr) vr >>= 1;
   while (vw > 1 && (TC * VS) % vw) vw >>= 1;
@@ -361,26 +362,34 @@ static int plan_tile(const Problem &p, bool shared, const TuneParams &tp,
 }
 
 void finalize_vectors(const Problem &p, LaunchPlan &lp, uint32_t vr, uint32_t vw) {
+  // vr / vw: the full width (16 / esize) where runs are exact (aligned, whole
+  // vectors); anything less selects unaligned full-width vectors with a
+  // partial last vector, or scalars for 16-byte elements.
+  const uint32_t cap = 16 / p.esize;
   if (lp.kind == kTile) {
     TileDesc &t = lp.tile;
-    t.vr = vr;
-    t.vw = vw;
-    t.nvr = (t.TR * t.VS) / vr;
-    t.nvw = (t.TC * t.VS) / vw;
+    t.ua_r = (cap > 1 && vr != cap) ? 1u : 0u;
+    t.ua_w = (cap > 1 && vw != cap) ? 1u : 0u;
+    t.vr = cap;
+    t.vw = cap;
+    t.nvr = (t.TR * t.VS + cap - 1) / cap;
+    t.nvw = (t.TC * t.VS + cap - 1) / cap;
     t.div_nvr = make_fastdiv(t.nvr);
     t.div_nvw = make_fastdiv(t.nvw);
     // LDS rows: a multiple of the read vector, padded by one vector so that
-    // the write pass's column gathers spread over banks.
+    // the write pass's column gathers spread over banks (odd for scalars).
     uint32_t row = t.TR * t.VS;
-    row = (row + vr - 1) / vr * vr;
-    t.lds_row = row + vr;
-    if (vr == 1 && (row % 2) == 0) t.lds_row = row + 1;
+    row = (row + cap - 1) / cap * cap;
+    t.lds_row = row + cap;
+    if (cap == 1 && (row % 2) == 0) t.lds_row = row + 1;
     lp.lds_bytes = tile_lds_bytes(t, p.esize);
     lp.grid = t.ntiles;
   } else {
     RowDesc &r = lp.rows;
-    r.v = vr;
-    r.nvs = (r.SEG + vr - 1) / vr;
+    const bool contiguous = r.ra == 1 && r.rb == 1;
+    r.v = contiguous ? cap : 1;
+    r.ua = (contiguous && cap > 1 && vr != cap) ? 1u : 0u;
+    r.nvs = (r.SEG + r.v - 1) / r.v;
     r.div_nvs = make_fastdiv(r.nvs);
     lp.lds_bytes = row_lds_bytes(r);
     lp.grid = r.ntiles;
@@ -416,8 +425,9 @@ int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
   if (rc != kOk) return rc;
   // Streaming mode.  Model: on when the problem cannot stay in the 256 MiB
   // last-level cache anyway and every access belongs to a run of at least
-  // 256 bytes on both sides (16-byte elements on the TILE path measured
-  // faster without it); the tuner may override.
+  // 128 bytes (tile runs) or 256 bytes (rows); exhaustive tuning of the
+  // large 6-D and 4-D workloads chose it for every winning tile shape.  The
+  // tuner may override.
   if (tp.stream == 1) {
     lp.stream = true;
   } else if (tp.stream == 2) {
@@ -426,8 +436,8 @@ int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
     const uint64_t bytes = p.N * p.esize * 2;
     bool long_runs;
     if (lp.kind == kTile)
-      long_runs = (uint64_t)lp.tile.TR * lp.tile.VS * p.esize >= 256 &&
-                  (uint64_t)lp.tile.TC * lp.tile.VS * p.esize >= 256 && p.esize < 16;
+      long_runs = (uint64_t)lp.tile.TR * lp.tile.VS * p.esize >= 128 &&
+                  (uint64_t)lp.tile.TC * lp.tile.VS * p.esize >= 128;
     else
       long_runs = lp.rows.ra == 1 && lp.rows.rb == 1 && (uint64_t)lp.rows.L * p.esize >= 256;
     lp.stream = long_runs && bytes > (256ull << 20);
@@ -449,12 +459,14 @@ std::string describe(const Problem &p, const LaunchPlan &lp) {
   if (lp.kind == kTile) {
     const TileDesc &t = lp.tile;
     os << "\ntile: VS=" << t.VS << " R=" << t.VR << "/" << t.TR << " C=" << t.VC << "/" << t.TC
-       << " O=" << t.VO << "/" << t.TO << " vec(r,w)=" << t.vr << "," << t.vw
+       << " O=" << t.VO << "/" << t.TO << " vec(r,w)=" << t.vr << (t.ua_r ? "u" : "") << ","
+       << t.vw << (t.ua_w ? "u" : "")
        << " tiles=" << t.ntiles << " lds=" << lp.lds_bytes << "B";
   } else {
     const RowDesc &r = lp.rows;
     os << "\nrows: L=" << r.L << " stride(a,b)=" << r.ra << "," << r.rb << " rows=" << r.nrows
-       << " seg=" << r.SEG << " rows/block=" << r.RB << " vec=" << r.v << " tiles=" << r.ntiles;
+       << " seg=" << r.SEG << " rows/block=" << r.RB << " vec=" << r.v << (r.ua ? "u" : "")
+       << " tiles=" << r.ntiles;
   }
   os << "\ngrid: " << lp.grid << " x " << kBlock << "  streaming: " << (lp.stream ? "on" : "off")
      << "  tune(run_r,run_c,block,wide,stream)=" << lp.tune.run_r << "," << lp.tune.run_c << ","

@@ -152,8 +152,9 @@ void hpttx_plan_destroy(hpttx_plan *plan);
  * Batched transpositions: many independent plans whose block tiles form one
  * queue, walked by persistent kernels (one launch per element type and
  * kernel variant, or per element type when HPTTX_BATCH_MODE=merged), captured
- * once in a hipGraph and replayed by hpttx_batch_exec.  No counterpart in the reference, whose only
- * way to run several transpositions is one plan->exec() after another
+ * once in a hipGraph and replayed by hpttx_batch_exec.  No counterpart in the
+ * reference, whose only way to run several transpositions is one
+ * plan->exec() after another
  * (src/hptt.cc:16-93, inc/hptt/cgraph/cgraph_trans.tcc:27-48); each item has
  * the meaning of one hpttx_plan_create_* call.  Tile shapes are chosen per
  * (dtype, fused rank, path, size class); with tuning_timeout != 0 each class

@@ -21,10 +21,23 @@ static Range g_A, g_B, g_L;
 static std::vector<uint8_t> *g_wcount = nullptr;
 static uint32_t g_esize = 4;
 static std::string g_fault;
+static void fault(const char *what, size_t n, bool st) {
+  if (g_fault.empty()) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "%s %s of %zu bytes", what, st ? "store" : "load", n);
+    g_fault = buf;
+  }
+}
 static void access(const void *p, size_t n, bool st) {
   const char *c = static_cast<const char *>(p);
   auto in = [&](const Range &r) { return c >= r.lo && c + n <= r.hi; };
-  if (in(g_L)) return;
+  if (in(g_L)) {
+    // LDS vector accesses must be naturally aligned
+    if ((size_t)(c - g_L.lo) % n) fault("misaligned LDS", n, st);
+    return;
+  }
+  // global accesses of a dword or more need only dword alignment
+  if (n >= 4 && (reinterpret_cast<uintptr_t>(c) % 4)) fault("non-dword-aligned global", n, st);
   if (!st && in(g_A)) return;
   if (in(g_B)) {
     if (st && g_wcount) {
@@ -37,11 +50,7 @@ static void access(const void *p, size_t n, bool st) {
     }
     return;
   }
-  if (g_fault.empty()) {
-    char buf[160];
-    std::snprintf(buf, sizeof buf, "out-of-bounds %s of %zu bytes", st ? "store" : "load", n);
-    g_fault = buf;
-  }
+  fault("out-of-bounds", n, st);
 }
 }  // namespace emu
 
@@ -54,7 +63,8 @@ using namespace hpttx;
 
 namespace {
 
-template <typename T, typename S, int NC, int VR, int VW, bool UPD, bool CC, bool SH>
+template <typename T, typename S, int NC, int VR, int VW, bool UR, bool UW, bool UPD, bool CC,
+          bool SH>
 void run_tile(const LaunchPlan &lp, const T *A, T *B, const Coef<S> &k) {
   const TileDesc &d = lp.tile;
   std::vector<uint64_t> smem((lp.lds_bytes + 7) / 8 + 2);
@@ -68,16 +78,17 @@ void run_tile(const LaunchPlan &lp, const T *A, T *B, const Coef<S> &k) {
     // per-thread state kept across the barrier, as in the kernel
     std::vector<WriteItems<T, VW, kUcapPlan>> first(kBlock);
     for (uint32_t tid = 0; tid < kBlock; ++tid) {
-      tile_write_setup<T, VW, UPD, false, kUcapPlan>(d, tc, tid, B, tb, first[tid]);
-      tile_read<T, VR, false, kUcapPlan>(d, tc, tid, A, lds, tb);
+      tile_write_setup<T, VW, UPD, false, kUcapPlan, UW>(d, tc, tid, B, tb, first[tid]);
+      tile_read<T, VR, false, kUcapPlan, UR>(d, tc, tid, A, lds, tb);
     }
     for (uint32_t tid = 0; tid < kBlock; ++tid)
-      tile_write<T, S, NC, VW, UPD, CC, SH, false, kUcapPlan>(d, tc, tid, B, lds, tb, k, &first[tid]);
+      tile_write<T, S, NC, VW, UPD, CC, SH, false, kUcapPlan, UW>(d, tc, tid, B, lds, tb, k,
+                                                                  &first[tid]);
     if (!emu::g_fault.empty()) return;
   }
 }
 
-template <typename T, typename S, int NC, int V, bool UPD, bool CC>
+template <typename T, typename S, int NC, int V, bool UA, bool UPD, bool CC>
 void run_rows(const LaunchPlan &lp, const T *A, T *B, const Coef<S> &k) {
   const RowDesc &d = lp.rows;
   std::vector<int64_t> tabA(d.RB), tabB(d.RB);
@@ -86,34 +97,39 @@ void run_rows(const LaunchPlan &lp, const T *A, T *B, const Coef<S> &k) {
     const RowCoord rc = row_coord(d, t);
     for (uint32_t tid = 0; tid < kBlock; ++tid) row_tables(d, rc, tid, tabA.data(), tabB.data());
     for (uint32_t tid = 0; tid < kBlock; ++tid)
-      row_body<T, S, NC, V, UPD, CC, false, kUcapPlan>(d, rc, tid, A, B, tabA.data(), tabB.data(), k);
+      row_body<T, S, NC, V, UPD, CC, false, kUcapPlan, UA>(d, rc, tid, A, B, tabA.data(),
+                                                           tabB.data(), k);
     if (!emu::g_fault.empty()) return;
   }
 }
 
+// The same variant choice as the library's launch (kernels.hip launch_typed).
 template <typename T, typename S, int NC, int VCAP, bool UPD, bool CC>
 void run_typed(const LaunchPlan &lp, const void *A, void *B, const Coef<S> &k) {
   const T *a = static_cast<const T *>(A);
   T *b = static_cast<T *>(B);
+  constexpr bool kUA = VCAP > 1;
   if (lp.kind == kTile) {
     const TileDesc &d = lp.tile;
-    const bool sh = d.VS > 1, vr = d.vr == VCAP && VCAP > 1, vw = d.vw == VCAP && VCAP > 1;
-#define EMU_TILE(VR_, VW_, SH_) run_tile<T, S, NC, VR_, VW_, UPD, CC, SH_>(lp, a, b, k)
+    const bool sh = d.VS > 1, ur = d.ua_r != 0, uw = d.ua_w != 0;
+#define EMU_TILE(UR_, UW_, SH_) run_tile<T, S, NC, VCAP, VCAP, UR_, UW_, UPD, CC, SH_>(lp, a, b, k)
     if (sh) {
-      if (vr && vw) EMU_TILE(VCAP, VCAP, true);
-      else if (vr) EMU_TILE(VCAP, 1, true);
-      else if (vw) EMU_TILE(1, VCAP, true);
-      else EMU_TILE(1, 1, true);
+      if (!ur && !uw) EMU_TILE(false, false, true);
+      else if (!ur) EMU_TILE(false, kUA, true);
+      else if (!uw) EMU_TILE(kUA, false, true);
+      else EMU_TILE(kUA, kUA, true);
     } else {
-      if (vr && vw) EMU_TILE(VCAP, VCAP, false);
-      else if (vr) EMU_TILE(VCAP, 1, false);
-      else if (vw) EMU_TILE(1, VCAP, false);
-      else EMU_TILE(1, 1, false);
+      if (!ur && !uw) EMU_TILE(false, false, false);
+      else if (!ur) EMU_TILE(false, kUA, false);
+      else if (!uw) EMU_TILE(kUA, false, false);
+      else EMU_TILE(kUA, kUA, false);
     }
 #undef EMU_TILE
   } else {
-    if (lp.rows.v == VCAP && VCAP > 1) run_rows<T, S, NC, VCAP, UPD, CC>(lp, a, b, k);
-    else run_rows<T, S, NC, 1, UPD, CC>(lp, a, b, k);
+    const RowDesc &d = lp.rows;
+    if (d.v == VCAP && VCAP > 1 && !d.ua) run_rows<T, S, NC, VCAP, false, UPD, CC>(lp, a, b, k);
+    else if (d.v == VCAP && VCAP > 1) run_rows<T, S, NC, VCAP, kUA, UPD, CC>(lp, a, b, k);
+    else run_rows<T, S, NC, 1, false, UPD, CC>(lp, a, b, k);
   }
 }
 
@@ -159,7 +175,9 @@ int emu_transpose(int dtype, const void *A, uint64_t nA, void *B, uint64_t nB, u
   LaunchPlan lp;
   rc = plan_launch(p, tp, lp, err);
   if (rc) { put(errbuf, errlen, err); return rc; }
-  // Same rule as the library's exec: full 16-byte vectors or scalars.
+  // Same rule as the library's exec: exact full vectors where the shape and
+  // the pointers' 16-byte alignment allow them, unaligned vectors otherwise
+  // (force_scalar: unaligned everywhere, scalars for 16-byte elements).
   const uint32_t cap = 16 / p.esize;
   uint32_t vr = lp.vr_shape, vw = lp.vw_shape;
   if (lp.kind == kRows) {

@@ -131,6 +131,40 @@ def test_integer_valued_exact(oracle_lib):
         assert np.array_equal(got, exp)
 
 
+def test_complex_coefficients_integer_exact(oracle_lib):
+    """Complex alpha/beta on integer-valued complex data: every product and
+    sum is exact, so the result must equal exact complex arithmetic
+    (alpha*x + beta*y in Python integers), whatever the rounding order.  This
+    fixes which components multiply and every sign; alpha = i additionally
+    reduces to (-x.im, x.re)."""
+    rng = np.random.default_rng(21)
+    for dt in (np.complex64, np.complex128):
+        size, perm = [5, 3, 4], [1, 2, 0]
+        n = int(np.prod(size))
+        re = lambda: rng.integers(-300, 300, n).astype(np.float64)
+        A = (re() + 1j * re()).astype(dt)
+        B = (re() + 1j * re()).astype(dt)
+        Af = A.reshape(size, order="F")
+        Bf = B.reshape([size[p] for p in perm], order="F")
+        for al, be in [(complex(3, -2), complex(-1, 4)), (1j, 0.0), (complex(0, -2), 1j)]:
+            got = B.copy()
+            assert oracle.transpose(A, got, size, perm, al, be, complex_coef=True) == 0
+            exp = np.empty(n, dtype=dt)
+            x = np.transpose(Af, perm).reshape(-1, order="F")
+            y = Bf.reshape(-1, order="F")
+            for k in range(n):
+                xr, xi = int(x[k].real), int(x[k].imag)
+                yr, yi = int(y[k].real), int(y[k].imag)
+                ar, ai, br, bi = int(al.real), int(al.imag), int(complex(be).real), int(complex(be).imag)
+                r = (ar * xr - ai * xi, ar * xi + ai * xr)
+                if abs(complex(be)) >= 1e-16:
+                    r = (r[0] + br * yr - bi * yi, r[1] + br * yi + bi * yr)
+                exp[k] = complex(*r)
+            assert np.array_equal(got, exp), (dt, al, be)
+            if al == 1j and be == 0.0:
+                assert np.array_equal(got, (-x.imag + 1j * x.real).astype(dt))
+
+
 def test_update_threshold(oracle_lib):
     assert oracle.update_real(1e-16)
     assert not oracle.update_real(9.999e-17)

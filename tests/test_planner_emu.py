@@ -74,10 +74,31 @@ def test_emulated_padded_and_misaligned(dt):
         size, perm = shapes.random_case(rng, max_elems=3000, max_dim=16)
         io, oo = shapes.random_outer(rng, size, perm)
         _check(dt, size, perm, 2.3, 4.2 if k % 2 else 0.0, io, oo)
-    # pointers one element off a 16-byte boundary force the scalar kernels
+    # pointers one element off a 16-byte boundary force the unaligned-vector
+    # kernels (scalars for 16-byte elements)
     for k in range(10):
         size, perm = shapes.random_case(rng, max_elems=3000)
-        _check(dt, size, perm, 1.5, 0.25, offset=1)
+        desc = _check(dt, size, perm, 1.5, 0.25, offset=1)
+        if np.dtype(dt).itemsize < 16 and "path: tile" in desc:
+            assert "vec(r,w)=%du,%du" % ((16 // np.dtype(dt).itemsize,) * 2) in desc, desc
+
+
+@pytest.mark.parametrize("dt", shapes.DTYPES)
+def test_emulated_unaligned_runs(dt):
+    """Odd extents (runs start at any element offset and end in partial
+    vectors) and the forced unaligned mode on every path, with and without
+    beta, against the oracle; every access in bounds and dword-aligned, every
+    element of B written once."""
+    rng = np.random.default_rng(99)
+    odd = [[7, 5, 3], [3, 9, 11, 2], [5, 7, 6, 3, 2], [13, 3], [3, 5, 7, 2, 3, 2],
+           [31, 33], [6, 10, 14]]
+    for size in odd:
+        for _ in range(3):
+            perm = [int(v) for v in rng.permutation(len(size))]
+            for be in (0.0, 4.2):
+                _check(dt, size, perm, 2.3, be)
+                _check(dt, size, perm, 2.3, be, force_scalar=True)
+                _check(dt, size, perm, 2.3, be, offset=1)
 
 
 @pytest.mark.parametrize("tune", [[8, 8, 256, 64], [16, 64, 1024, 128], [128, 16, 4096, 1024],
