@@ -38,11 +38,9 @@ __device__ __forceinline__ void tile_one(const TileDesc &d, uint32_t lt, uint32_
   const TileCoord tc = tile_coord(d, lt);
   tile_tables(d, tc, tid, tb);
   __syncthreads();
-  WriteItems<T, VW, UCAP> first;
-  tile_write_setup<T, VW, UPD, NT, UCAP, UW>(d, tc, tid, B, tb, first);
   tile_read<T, VR, NT, UCAP, UR>(d, tc, tid, A, lds, tb);
   __syncthreads();
-  tile_write<T, S, NC, VW, UPD, CC, SH, NT, UCAP, UW>(d, tc, tid, B, lds, tb, co, &first);
+  tile_write<T, S, NC, VW, UPD, CC, SH, NT, UCAP, UW>(d, tc, tid, B, lds, tb, co);
   __syncthreads();
 }
 

@@ -90,7 +90,7 @@ int create_common(int dtype, const void *in, void *out, uint32_t rank,
       return fail(rc, err);
     }
   }
-  rc = plan_launch(p->prob, tp, p->shape, err);
+  rc = plan_launch(p->prob, tp, p->shape, err, kMinPlanTiles);
   if (rc != kOk) {
     delete p;
     return fail(rc, err);

@@ -389,10 +389,10 @@ HPTTX_HD void tile_read(const TileDesc &d, const TileCoord &tc, uint32_t tid,
 
 // Write pass: item k = (row, v), row = o*TR + r; VW elements of B's run
 // c0*VS + [0, clen*VS) gathered from LDS column r, combined with alpha/beta,
-// written as one vector.  Split in two so that a thread's first U items can
-// be set up -- and, when B is read, B's values loaded -- before the read pass,
-// overlapping the B-read latency with the A-read latency.  Only the item
-// index and B's values are kept between the halves; addresses are recomputed.
+// written as one vector.  A thread's U items are set up (and, when B is
+// read, B's values loaded) first, then gathered, combined and stored.  (B is
+// not prefetched before the read pass: holding its values across the read
+// pass costs occupancy, and measured slower on MI355X.)
 // UA: runs start at any element offset and may end in a partial vector,
 // which is read, gathered and written element by element.
 template <typename T, int VW, int UCAP>
@@ -487,21 +487,15 @@ HPTTX_HD void tile_write_finish(const TileDesc &d, const TileCoord &tc, T *HPTTX
   }
 }
 
-// The whole write pass; `first` holds the thread's first items already set up
-// by tile_write_setup(k0 = tid) (with B prefetched), or is null.
+// The whole write pass.
 template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT, int UCAP,
           bool UA = false>
 HPTTX_HD void tile_write(const TileDesc &d, const TileCoord &tc, uint32_t tid,
                          T *HPTTX_RESTRICT B, const T *lds, const TileTabs &tb,
-                         const Coef<S> &k, const WriteItems<T, VW, UCAP> *first) {
+                         const Coef<S> &k) {
   constexpr int U = WriteItems<T, VW, UCAP>::U;
   const uint32_t items = d.nvw * d.TR * tc.olen;
-  uint32_t k0 = tid;
-  if (first) {
-    tile_write_finish<T, S, NC, VW, UPD, CC, SH, NT, UCAP, UA>(d, tc, B, lds, tb, *first, k);
-    k0 += kBlock * U;
-  }
-  for (; k0 < items; k0 += kBlock * U) {
+  for (uint32_t k0 = tid; k0 < items; k0 += kBlock * U) {
     WriteItems<T, VW, UCAP> w;
     tile_write_setup<T, VW, UPD, NT, UCAP, UA>(d, tc, k0, B, tb, w);
     tile_write_finish<T, S, NC, VW, UPD, CC, SH, NT, UCAP, UA>(d, tc, B, lds, tb, w, k);

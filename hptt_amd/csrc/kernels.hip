@@ -23,8 +23,7 @@
 
 namespace hpttx {
 
-// One block per tile on a 2-D grid (t = y * gridDim.x + x).  The write pass's
-// first items are set up, and B prefetched, before the read pass.  UR / UW:
+// One block per tile on a 2-D grid (t = y * gridDim.x + x).  UR / UW:
 // unaligned runs on the read / write side (full-width vectors at any element
 // offset, partial last vectors element by element).
 template <typename T, typename S, int NC, int VR, int VW, bool UR, bool UW, bool UPD, bool CC,
@@ -40,11 +39,9 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileDesc d, const T *__restrict
   const TileCoord tc = tile_coord(d, t);
   tile_tables(d, tc, tid, tb);
   __syncthreads();
-  WriteItems<T, VW, kUcapPlan> first;
-  tile_write_setup<T, VW, UPD, NT, kUcapPlan, UW>(d, tc, tid, B, tb, first);
   tile_read<T, VR, NT, kUcapPlan, UR>(d, tc, tid, A, lds, tb);
   __syncthreads();
-  tile_write<T, S, NC, VW, UPD, CC, SH, NT, kUcapPlan, UW>(d, tc, tid, B, lds, tb, co, &first);
+  tile_write<T, S, NC, VW, UPD, CC, SH, NT, kUcapPlan, UW>(d, tc, tid, B, lds, tb, co);
 }
 
 template <typename T, typename S, int NC, int V, bool UA, bool UPD, bool CC, bool NT>

@@ -118,12 +118,12 @@ static bool divisible(int64_t x, uint32_t v) { return (x % (int64_t)v) == 0; }
 
 TuneParams default_tune(const Problem &p) {
   TuneParams t;
-  // Read runs of 32 elements along A, write runs of 512 bytes along B (256
-  // for 16-byte elements) and ~8 KiB blocks: what exhaustive tuning of the
-  // 6-D fp32/fp64 and 4-D complex<double> workloads picked on MI355X
-  // (32x128, 32x64 and 32x16 tiles).
-  t.run_r = 32;
-  t.run_c = p.esize >= 16 ? 16 : 512 / p.esize;
+  // Read runs of 256 bytes along A, write runs of 512 bytes along B, and
+  // tiles of at least ~4 KiB: what exhaustive tuning of the 6-D fp32 / fp64
+  // and 4-D complex<double> workloads picked on MI355X (64x128, 32x64 and
+  // 16x32 tiles).
+  t.run_r = 256 / p.esize;
+  t.run_c = 512 / p.esize;
   t.block_elems = 8192 / p.esize;
   t.wide_min_bytes = 256;
   return t;
@@ -206,7 +206,7 @@ static int plan_rows(const Problem &p, int rowdim, const TuneParams &tp,
   return kOk;
 }
 
-static int plan_tile(const Problem &p, bool shared, const TuneParams &tp,
+static int plan_tile(const Problem &p, bool shared, const TuneParams &tp, uint32_t min_tiles,
                      LaunchPlan &lp, std::string &err) {
   TileDesc &t = lp.tile;
   t = TileDesc();
@@ -307,20 +307,9 @@ static int plan_tile(const Problem &p, bool shared, const TuneParams &tp,
   // then, while the tile holds less than half of block_elems, lengthen the
   // shorter run (doubling) as far as its super-dim allows; the O count fills
   // the rest.  Each extent is then balanced over its tile count (no sliver
-  // edge tiles).
+  // edge tiles) and rounded to whole 16-byte vectors (interior tiles then
+  // need no partial vector even where the runs are unaligned).
   auto cdiv = [](uint64_t a, uint64_t b) { return (a + b - 1) / b; };
-  uint64_t TR = std::min<uint64_t>(VR, std::max<uint64_t>(1, cdiv(tp.run_r, VS)));
-  uint64_t TC = std::min<uint64_t>(VC, std::max<uint64_t>(1, cdiv(tp.run_c, VS)));
-  const uint64_t half_block = std::max<uint64_t>(1, tp.block_elems / 2);
-  while (TR * TC * VS < half_block) {
-    const bool can_r = TR < VR, can_c = TC < VC;
-    if (!can_r && !can_c) break;
-    const bool grow_r = can_r && (!can_c || TR <= TC);
-    if (grow_r) TR = std::min<uint64_t>(VR, TR * 2);
-    else TC = std::min<uint64_t>(VC, TC * 2);
-  }
-  // Extents are rounded to whole 16-byte vectors (interior tiles then need
-  // no partial vector even where the runs are unaligned).
   auto balance = [&](uint64_t V, uint64_t T, uint32_t vec) {
     const uint64_t n = cdiv(V, T);
     uint64_t b = cdiv(V, n);
@@ -330,15 +319,43 @@ static int plan_tile(const Problem &p, bool shared, const TuneParams &tp,
     if (g > 1) b = std::min<uint64_t>(V, cdiv(b, g) * g);
     return std::max<uint64_t>(1, b);
   };
-  TR = balance(VR, TR, vcap);
-  TC = balance(VC, TC, vcap);
-  // Whole-tile runs must stay multiples of the vector widths.
+  uint64_t TR = 1, TC = 1, TO = 1;
+  auto size_tile = [&](uint32_t run_r, uint32_t run_c, uint32_t block) {
+    TR = std::min<uint64_t>(VR, std::max<uint64_t>(1, cdiv(run_r, VS)));
+    TC = std::min<uint64_t>(VC, std::max<uint64_t>(1, cdiv(run_c, VS)));
+    const uint64_t half_block = std::max<uint64_t>(1, block / 2);
+    while (TR * TC * VS < half_block) {
+      const bool can_r = TR < VR, can_c = TC < VC;
+      if (!can_r && !can_c) break;
+      const bool grow_r = can_r && (!can_c || TR <= TC);
+      if (grow_r) TR = std::min<uint64_t>(VR, TR * 2);
+      else TC = std::min<uint64_t>(VC, TC * 2);
+    }
+    TR = balance(VR, TR, vcap);
+    TC = balance(VC, TC, vcap);
+    const uint64_t per_o = TR * VS * TC;
+    TO = std::max<uint64_t>(1, block / std::max<uint64_t>(per_o, 1));
+    TO = std::min<uint64_t>(std::min<uint64_t>(TO, VO), 512);
+    TO = cdiv(VO, cdiv(VO, TO));
+    return cdiv(VR, TR) * cdiv(VC, TC) * cdiv(VO, TO);
+  };
+  uint32_t run_r = tp.run_r, run_c = tp.run_c, block = tp.block_elems;
+  uint64_t count = size_tile(run_r, run_c, block);
+  // Small problems: halve the runs (read runs down to 128 bytes, write runs
+  // down to 256) until the launch has min_tiles blocks to spread over the
+  // CUs.
+  while (count < min_tiles) {
+    const uint64_t rb = (uint64_t)run_r * p.esize, cb = (uint64_t)run_c * p.esize;
+    const bool can_r = rb > 128 && TR * VS > 1, can_c = cb > 256 && TC * VS > 1;
+    if (!can_r && !can_c) break;
+    if (can_r && (!can_c || rb >= cb / 2)) run_r /= 2;
+    else run_c /= 2;
+    block = std::max<uint32_t>(kBlock, block / 2);
+    count = size_tile(run_r, run_c, block);
+  }
+  // Whole-tile runs must stay multiples of the exact vector widths.
   while (vr > 1 && (TR * VS) % vr) vr >>= 1;
   while (vw > 1 && (TC * VS) % vw) vw >>= 1;
-  const uint64_t per_o = TR * VS * TC;
-  uint64_t TO = std::max<uint64_t>(1, tp.block_elems / std::max<uint64_t>(per_o, 1));
-  TO = std::min<uint64_t>(std::min<uint64_t>(TO, VO), 512);
-  TO = cdiv(VO, cdiv(VO, TO));
   t.TR = (uint32_t)TR;
   t.TC = (uint32_t)TC;
   t.TO = (uint32_t)TO;
@@ -398,7 +415,7 @@ void finalize_vectors(const Problem &p, LaunchPlan &lp, uint32_t vr, uint32_t vw
 }
 
 int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
-                std::string &err) {
+                std::string &err, uint32_t min_tiles) {
   TuneParams tp = tp_in;
   fill_tune_defaults(p, tp);
   lp = LaunchPlan();
@@ -418,9 +435,9 @@ int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
     if (L * p.esize >= tp.wide_min_bytes || !contiguous_after || p.border[1] == 1)
       rc = plan_rows(p, 0, tp, lp, err);
     else
-      rc = plan_tile(p, true, tp, lp, err);
+      rc = plan_tile(p, true, tp, min_tiles, lp, err);
   } else {
-    rc = plan_tile(p, false, tp, lp, err);
+    rc = plan_tile(p, false, tp, min_tiles, lp, err);
   }
   if (rc != kOk) return rc;
   // Streaming mode.  Model: on when the problem cannot stay in the 256 MiB

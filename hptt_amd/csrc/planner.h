@@ -80,12 +80,20 @@ int build_problem(int dtype, uint32_t rank, const uint32_t *in_size,
                   const uint32_t *perm, const uint32_t *in_outer,
                   const uint32_t *out_outer, Problem &p, std::string &err);
 
-// Shape plan for the widest legal vectors (capped by max_vec_bytes).
-int plan_launch(const Problem &p, const TuneParams &tp, LaunchPlan &lp,
-                std::string &err);
+// Blocks a single plan's launch should have at least (4 per CU): small
+// problems get shorter tile runs rather than an under-filled GPU.  Batches
+// pass 0 (their launches are filled by many transpositions).
+constexpr uint32_t kMinPlanTiles = 1024;
 
-// Re-derive the vector-dependent fields of lp for vector widths (vr, vw);
-// both must divide what the shape allows.
+// Shape plan (path, tile or row decomposition, exact-vector widths the
+// strides allow); tile runs shrink until there are min_tiles tiles or the
+// runs reach 128 / 256 bytes.
+int plan_launch(const Problem &p, const TuneParams &tp, LaunchPlan &lp,
+                std::string &err, uint32_t min_tiles = 0);
+
+// Re-derive the vector-dependent fields of lp: vr / vw equal to the full
+// width 16 / esize select exact vectors on that side, anything less
+// unaligned full-width vectors (scalars for 16-byte elements).
 void finalize_vectors(const Problem &p, LaunchPlan &lp, uint32_t vr, uint32_t vw);
 
 // Widest vector (elements) a pointer's alignment allows for this element size.

@@ -139,7 +139,7 @@ int autotune(const Problem &p, const CoefD &coef, bool update, bool cc, double t
       }
       LaunchPlan lp;
       std::string e2;
-      if (plan_launch(p, t, lp, e2) != kOk) continue;
+      if (plan_launch(p, t, lp, e2, kMinPlanTiles) != kOk) continue;
       bool dup = false;
       for (const LaunchPlan &s2 : seen) dup = dup || same_launch(s2, lp);
       if (dup) continue;
@@ -163,7 +163,7 @@ int autotune(const Problem &p, const CoefD &coef, bool update, bool cc, double t
     for (size_t i = 0; i < scored.size() && i < 3; ++i) {
       LaunchPlan lp;
       std::string e2;
-      if (plan_launch(p, scored[i].second, lp, e2) != kOk) continue;
+      if (plan_launch(p, scored[i].second, lp, e2, kMinPlanTiles) != kOk) continue;
       float ms = 0;
       if (!time_plan(lp, 10, ms)) {
         err = "autotune: event sync failed";

@@ -75,15 +75,10 @@ void run_tile(const LaunchPlan &lp, const T *A, T *B, const Coef<S> &k) {
   for (uint32_t t = 0; t < d.ntiles; ++t) {
     const TileCoord tc = tile_coord(d, t);
     for (uint32_t tid = 0; tid < kBlock; ++tid) tile_tables(d, tc, tid, tb);
-    // per-thread state kept across the barrier, as in the kernel
-    std::vector<WriteItems<T, VW, kUcapPlan>> first(kBlock);
-    for (uint32_t tid = 0; tid < kBlock; ++tid) {
-      tile_write_setup<T, VW, UPD, false, kUcapPlan, UW>(d, tc, tid, B, tb, first[tid]);
-      tile_read<T, VR, false, kUcapPlan, UR>(d, tc, tid, A, lds, tb);
-    }
     for (uint32_t tid = 0; tid < kBlock; ++tid)
-      tile_write<T, S, NC, VW, UPD, CC, SH, false, kUcapPlan, UW>(d, tc, tid, B, lds, tb, k,
-                                                                  &first[tid]);
+      tile_read<T, VR, false, kUcapPlan, UR>(d, tc, tid, A, lds, tb);
+    for (uint32_t tid = 0; tid < kBlock; ++tid)
+      tile_write<T, S, NC, VW, UPD, CC, SH, false, kUcapPlan, UW>(d, tc, tid, B, lds, tb, k);
     if (!emu::g_fault.empty()) return;
   }
 }
@@ -173,7 +168,8 @@ int emu_transpose(int dtype, const void *A, uint64_t nA, void *B, uint64_t nB, u
   TuneParams tp{};
   if (tune) tp = TuneParams{tune[0], tune[1], tune[2], tune[3], 0};
   LaunchPlan lp;
-  rc = plan_launch(p, tp, lp, err);
+  // explicit tile parameters are taken as given (no small-problem shrink)
+  rc = plan_launch(p, tp, lp, err, tune ? 0 : kMinPlanTiles);
   if (rc) { put(errbuf, errlen, err); return rc; }
   // Same rule as the library's exec: exact full vectors where the shape and
   // the pointers' 16-byte alignment allow them, unaligned vectors otherwise
@@ -248,7 +244,7 @@ int emu_plan(int dtype, uint32_t rank, const uint32_t *in_size, const uint32_t *
   TuneParams tp{};
   if (tune) tp = TuneParams{tune[0], tune[1], tune[2], tune[3], 0};
   LaunchPlan lp;
-  rc = plan_launch(p, tp, lp, err);
+  rc = plan_launch(p, tp, lp, err, kMinPlanTiles);
   if (rc) { put(errbuf, errlen, err); return rc; }
   put(desc, desclen, describe(p, lp));
   return 0;
