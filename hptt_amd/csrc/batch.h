@@ -1,15 +1,13 @@
-// batch.h -- many independent transpositions in one persistent launch.
+// batch.h -- many independent transpositions in a few graph-captured launches.
 //
 // No counterpart in the reference (one plan per transposition, run one after
 // another: src/hptt.cc:16-93 + cgraph_trans.tcc:27-48).  Each transposition
 // is planned like a single plan (planner.h); its block tiles are numbered
-// consecutively, largest transposition first, and a persistent kernel per
-// launch group walks the tile numbers in a static round-robin order (block b
-// takes tiles b, b + grid, b + 2 grid, ...), finding each tile's
-// transposition by a binary search over the entries' first tiles.  The order
-// needs no work counter and keeps the blocks sweeping memory in step.  The
-// launches are captured in a hipGraph once and replayed.  The serving loop
-// (serve.h) drives the same kernel with per-launch tables.
+// consecutively, largest transposition first, and one kernel launch per
+// group runs one block per tile (in an XCD-aware order), finding each
+// tile's transposition by a binary search over the entries' first tiles.
+// The launches are captured in a hipGraph once and replayed.  The serving
+// loop (serve.h) drives the same kernel with per-launch tables.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -69,18 +67,31 @@ int plan_item(const BatchItemSpec &it, const TuneParams &tp, PlannedItem &out, s
 std::string class_key(const PlannedItem &q);
 // Fill a device entry for a planned item (tile_begin left to the caller).
 void make_entry(const PlannedItem &q, const BatchItemSpec &it, BatchEntry &e);
-// LDS per block a batch launch uses for entries needing lds_bytes (rounded
-// up to 4 KiB).
+// LDS per block a batch launch uses for entries needing lds_bytes (a
+// 16-byte multiple; a launch takes the largest of its entries').
 uint32_t batch_lds_bytes(uint32_t lds_bytes);
-// Persistent blocks for one launch of the (dtype, variant) kernel with this
-// LDS per block (occupancy looked up once per kernel and LDS size, cached).
+// Blocks of one launch.  The kernel is a work-queue walker over the
+// launch's tiles (block b takes slots b, b + grid, ...).  By default it is
+// launched with one block per tile, so the hardware dispatcher hands out the
+// queue in order as blocks retire: measured faster on MI355X than a
+// persistent grid, whose blocks drift apart and lose the in-order sweep
+// (fp64 manifest of 4096 tensors: 9.2 ms against 10.6 ms).  With
+// HPTTX_BATCH_GRID=persistent the grid is the resident-block count
+// (occupancy of the (dtype, variant) kernel at lds_bytes, times the CUs;
+// looked up once and cached).
+bool batch_persistent();
 uint32_t batch_grid(int dtype, uint32_t variant, uint32_t lds_bytes, uint32_t total_tiles);
-// Launch the persistent kernel over entries [0, nent) whose tiles number
-// `total`.  `variant` is a variant code (every entry has it) or
-// batch_group_key(., true) for the instance that dispatches per entry.
+// Launch the batch kernel over entries [0, nent) whose tiles number
+// `total`, on `grid` blocks (a 2-D grid beyond 2^20 blocks).  `variant` is
+// a variant code (every entry has it) or batch_group_key(., true) for the
+// instance that dispatches per entry.
+// d_hints: null, or the entry holding tile 64 j for every j plus a final
+// nent - 1 (batch_hints), which narrows each block's entry search.
 hipError_t launch_batch_group(int dtype, uint32_t variant, const BatchEntry *d_ent,
-                              const uint32_t *d_begins, uint32_t nent, uint32_t total,
-                              uint32_t lds_bytes, uint32_t grid, hipStream_t st);
+                              const uint32_t *d_begins, const uint32_t *d_hints, uint32_t nent,
+                              uint32_t total, uint32_t lds_bytes, uint32_t grid, hipStream_t st);
+// Search hints for entries whose first tiles are `begins` (total tiles).
+std::vector<uint32_t> batch_hints(const std::vector<uint32_t> &begins, uint32_t total);
 // Kernel key of an entry's group: its variant, or the per-type instance
 // that dispatches variants at run time when `merged`.
 uint32_t batch_group_key(uint32_t variant, bool merged);
@@ -100,6 +111,7 @@ struct BatchGroup {
   uint32_t grid = 0;
   BatchEntry *d_ent = nullptr;      // device table
   uint32_t *d_begins = nullptr;     // device: first tile of each entry
+  uint32_t *d_hints = nullptr;      // device: batch_hints of the group
   std::vector<BatchEntry> h_ent;    // host copy
   std::vector<uint32_t> order;      // entry k <- user item order[k]
   uint64_t bytes = 0;               // algorithmic bytes
@@ -124,7 +136,7 @@ struct Batch {
 // Plan every item (tile shapes tuned per class when tuning_timeout != 0).
 int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, double tuning_timeout,
                 Batch &b, std::string &err);
-// Enqueue one persistent kernel per group, forked over the side streams.
+// Enqueue one kernel launch per group, forked over the side streams.
 hipError_t batch_enqueue(Batch &b, hipStream_t stream);
 // Capture batch_enqueue into a graph (done on first exec).
 hipError_t batch_capture(Batch &b);

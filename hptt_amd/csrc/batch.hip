@@ -1,5 +1,5 @@
 // batch.hip -- host side of batched transpositions: planning the items,
-// building the launch groups and their device tables, persistent grids,
+// building the launch groups and their device tables, launch grids,
 // launches over side streams and the captured hipGraph.  The kernels are in
 // batch_kernels.h (instantiated by batch_kernels_<type>.hip).
 #include <hip/hip_runtime.h>
@@ -113,18 +113,20 @@ void make_entry(const PlannedItem &q, const BatchItemSpec &it, BatchEntry &e) {
   else e.u.rows = q.lp.rows;
 }
 
-uint32_t batch_lds_bytes(uint32_t lds_bytes) {
-  // 4 KiB granules: few distinct launch shapes, so occupancy is looked up
-  // (and cached) for at most 16 sizes per element type.
-  return (lds_bytes + 4095u) & ~4095u;
+uint32_t batch_lds_bytes(uint32_t lds_bytes) { return (lds_bytes + 15u) & ~15u; }
+
+bool batch_persistent() {
+  const char *m = std::getenv("HPTTX_BATCH_GRID");
+  return m && std::strcmp(m, "persistent") == 0;
 }
 
 uint32_t batch_grid(int dtype, uint32_t variant, uint32_t lds_bytes, uint32_t total_tiles) {
+  total_tiles = std::max<uint32_t>(1, total_tiles);
+  if (!batch_persistent()) return total_tiles;
   static std::mutex mu;
   static std::map<std::tuple<int, uint32_t, uint32_t>, uint32_t> cache;
   static int ncu = 0;
   uint32_t per_gpu;
-  lds_bytes = batch_lds_bytes(lds_bytes);
   {
     std::lock_guard<std::mutex> lk(mu);
     if (!ncu) {
@@ -150,17 +152,34 @@ uint32_t batch_grid(int dtype, uint32_t variant, uint32_t lds_bytes, uint32_t to
       cache[key] = per_gpu;
     }
   }
-  return std::max<uint32_t>(1, std::min<uint32_t>(per_gpu, std::max<uint32_t>(1, total_tiles)));
+  return std::min<uint32_t>(per_gpu, total_tiles);
+}
+
+std::vector<uint32_t> batch_hints(const std::vector<uint32_t> &begins, uint32_t total) {
+  const size_t nh = ((size_t)total + 63) / 64 + 1;
+  std::vector<uint32_t> h(nh, 0);
+  const uint32_t nent = (uint32_t)begins.size();
+  uint32_t e = 0;
+  for (size_t j = 0; j + 1 < nh; ++j) {
+    const uint64_t t = (uint64_t)j * 64;
+    while (e + 1 < nent && begins[e + 1] <= t) ++e;
+    h[j] = e;
+  }
+  h[nh - 1] = nent ? nent - 1 : 0;
+  return h;
 }
 
 hipError_t launch_batch_group(int dtype, uint32_t variant, const BatchEntry *d_ent,
-                              const uint32_t *d_begins, uint32_t nent, uint32_t total,
-                              uint32_t lds_bytes, uint32_t grid, hipStream_t st) {
+                              const uint32_t *d_begins, const uint32_t *d_hints, uint32_t nent,
+                              uint32_t total, uint32_t lds_bytes, uint32_t grid, hipStream_t st) {
   if (!total || !nent) return hipSuccess;
   const void *k = batch_kernel(dtype, variant);
   if (!k) return hipErrorInvalidValue;
-  void *args[] = {(void *)&d_ent, (void *)&d_begins, (void *)&nent, (void *)&total};
-  return hipLaunchKernel(k, dim3(grid), dim3(kBlock), args, lds_bytes, st);
+  void *args[] = {(void *)&d_ent, (void *)&d_begins, (void *)&d_hints, (void *)&nent,
+                  (void *)&total};
+  // 2-D grid: x * 256 threads must stay below 2^32 work items per dimension.
+  const uint32_t gx = grid < kGridX ? grid : kGridX;
+  return hipLaunchKernel(k, dim3(gx, (grid + gx - 1) / gx), dim3(kBlock), args, lds_bytes, st);
 }
 
 uint32_t batch_group_key(uint32_t variant, bool merged) { return merged ? (uint32_t)kVarCodes : variant; }
@@ -211,18 +230,23 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
   // (measured faster than one dispatching launch per type on the 4096-tensor
   // mix: 6.7 ms vs 8.2 ms fp32).
   const bool merged = batch_merged_default(false);
-  std::map<std::pair<int, uint32_t>, std::vector<size_t>> by_kernel;
+  // Groups: (element type, kernel, LDS class).  A launch reserves its
+  // largest entry's LDS for every block, so entries are also split by LDS
+  // need (8 KiB classes): small tiles keep their occupancy.
+  std::map<std::tuple<int, uint32_t, uint32_t>, std::vector<size_t>> by_kernel;
   for (size_t k = 0; k < items.size(); ++k) {
     b.paths[k] = pl[k].lp.path;
     BatchEntry probe;
     make_entry(pl[k], items[k], probe);
-    by_kernel[{items[k].dtype, batch_group_key(probe.variant, merged)}].push_back(k);
+    by_kernel[std::make_tuple(items[k].dtype, batch_group_key(probe.variant, merged),
+                              (probe.lds_bytes + 8191u) / 8192u)]
+        .push_back(k);
   }
   hipError_t he = hipSuccess;
   for (auto &kv : by_kernel) {
     BatchGroup g;
-    g.dtype = kv.first.first;
-    g.variant = kv.first.second;
+    g.dtype = std::get<0>(kv.first);
+    g.variant = std::get<1>(kv.first);
     std::vector<size_t> idx = kv.second;
     // Largest transposition first: its tiles are handed out first and the
     // small ones fill the tail.
@@ -247,12 +271,17 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
     g.total_tiles = (uint32_t)tiles;
     g.lds_bytes = batch_lds_bytes(g.lds_bytes);
     g.grid = batch_grid(g.dtype, g.variant, g.lds_bytes, g.total_tiles);
+    const std::vector<uint32_t> hints = batch_hints(begins, g.total_tiles);
     he = hipMalloc(&g.d_ent, sizeof(BatchEntry) * g.nent);
     if (he == hipSuccess) he = hipMalloc(&g.d_begins, sizeof(uint32_t) * g.nent);
+    if (he == hipSuccess) he = hipMalloc(&g.d_hints, sizeof(uint32_t) * hints.size());
     if (he == hipSuccess)
       he = hipMemcpy(g.d_ent, g.h_ent.data(), sizeof(BatchEntry) * g.nent, hipMemcpyHostToDevice);
     if (he == hipSuccess)
       he = hipMemcpy(g.d_begins, begins.data(), sizeof(uint32_t) * g.nent, hipMemcpyHostToDevice);
+    if (he == hipSuccess)
+      he = hipMemcpy(g.d_hints, hints.data(), sizeof(uint32_t) * hints.size(),
+                     hipMemcpyHostToDevice);
     b.bytes += g.bytes;
     b.groups.push_back(std::move(g));
     if (he != hipSuccess) break;
@@ -304,7 +333,8 @@ hipError_t batch_enqueue(Batch &b, hipStream_t st) {
   for (size_t i = 0; i < b.groups.size() && e == hipSuccess; ++i) {
     const BatchGroup &g = b.groups[i];
     hipStream_t ls = b.lane[i] == 0 ? st : b.side[b.lane[i] - 1];
-    e = launch_batch_group(g.dtype, g.variant, g.d_ent, g.d_begins, g.nent, g.total_tiles,
+    e = launch_batch_group(g.dtype, g.variant, g.d_ent, g.d_begins, g.d_hints, g.nent,
+                           g.total_tiles,
                            g.lds_bytes, g.grid, ls);
   }
   for (size_t i = 0; i < b.side.size() && e == hipSuccess; ++i) {
@@ -341,6 +371,8 @@ void batch_free(Batch &b) {
   for (BatchGroup &g : b.groups) {
     if (g.d_ent) (void)hipFree(g.d_ent);
     if (g.d_begins) (void)hipFree(g.d_begins);
+    if (g.d_hints) (void)hipFree(g.d_hints);
+    g.d_hints = nullptr;
     g.d_ent = nullptr;
     g.d_begins = nullptr;
   }

@@ -1,4 +1,4 @@
-// batch_kernels.h -- the persistent batch kernel templates (device code).
+// batch_kernels.h -- the batch kernel templates (device code).
 //
 // Included by one source file per element type (batch_kernels_<type>.hip),
 // each of which instantiates every variant of its type and fills a table of
@@ -32,7 +32,7 @@ template <typename T, typename S, int NC, int VR, int VW, bool UR, bool UW, bool
           bool SH, bool NT, int UCAP>
 __device__ __forceinline__ void tile_one(const TileDesc &d, uint32_t lt, uint32_t tid,
                                          const T *A, T *B, unsigned char *smem,
-                                         const Coef<S> &co) {
+                                         const Coef<S> &co, bool more) {
   T *lds = reinterpret_cast<T *>(smem);
   const TileTabs tb = tile_tabs(d, smem, sizeof(T));
   const TileCoord tc = tile_coord(d, lt);
@@ -41,20 +41,20 @@ __device__ __forceinline__ void tile_one(const TileDesc &d, uint32_t lt, uint32_
   tile_read<T, VR, NT, UCAP, UR>(d, tc, tid, A, lds, tb);
   __syncthreads();
   tile_write<T, S, NC, VW, UPD, CC, SH, NT, UCAP, UW>(d, tc, tid, B, lds, tb, co);
-  __syncthreads();
+  if (more) __syncthreads();   // LDS is reused by the block's next tile
 }
 
 template <typename T, typename S, int NC, int V, bool UA, bool UPD, bool CC, bool NT, int UCAP>
 __device__ __forceinline__ void rows_one(const RowDesc &d, uint32_t lt, uint32_t tid,
                                          const T *A, T *B, unsigned char *smem,
-                                         const Coef<S> &co) {
+                                         const Coef<S> &co, bool more) {
   int64_t *tabA = reinterpret_cast<int64_t *>(smem);
   int64_t *tabB = tabA + d.RB;
   const RowCoord rc = row_coord(d, lt);
   row_tables(d, rc, tid, tabA, tabB);
   __syncthreads();
   row_body<T, S, NC, V, UPD, CC, NT, UCAP, UA>(d, rc, tid, A, B, tabA, tabB, co);
-  __syncthreads();
+  if (more) __syncthreads();
 }
 
 // Variant codes that exist for an element type.  Tiles: kVarVr / kVarVw mark
@@ -75,7 +75,7 @@ constexpr bool valid_code(int c) {
 // counts for more than depth in flight across many small transpositions).
 template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE, int UCAP>
 __device__ __forceinline__ void run_code(const BatchEntry &e, uint32_t lt, uint32_t tid,
-                                         unsigned char *smem) {
+                                         unsigned char *smem, bool more) {
   constexpr bool rows = CODE & kVarRows;
   constexpr bool exact_r = CODE & kVarVr;
   constexpr bool exact_w = CODE & kVarVw;
@@ -89,33 +89,32 @@ __device__ __forceinline__ void run_code(const BatchEntry &e, uint32_t lt, uint3
   if constexpr (rows) {
     // exact_w on rows: unaligned vectors
     constexpr int v = (exact_r || exact_w) ? VCAP : 1;
-    rows_one<T, S, NC, v, exact_w, upd, cc, nt, UCAP>(e.u.rows, lt, tid, A, B, smem, co);
+    rows_one<T, S, NC, v, exact_w, upd, cc, nt, UCAP>(e.u.rows, lt, tid, A, B, smem, co, more);
   } else {
     constexpr bool ua = VCAP > 1;
     tile_one<T, S, NC, VCAP, VCAP, ua && !exact_r, ua && !exact_w, upd, cc, sh, nt, UCAP>(
-        e.u.tile, lt, tid, A, B, smem, co);
+        e.u.tile, lt, tid, A, B, smem, co, more);
   }
 }
 
 template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
 __device__ __forceinline__ void dispatch(uint32_t v, const BatchEntry &e, uint32_t lt,
-                                         uint32_t tid, unsigned char *smem) {
+                                         uint32_t tid, unsigned char *smem, bool more) {
   if constexpr (CODE < kVarCodes) {
     if constexpr (valid_code<VCAP, CCOK>(CODE)) {
       if (v == CODE) {
-        run_code<T, S, NC, VCAP, CCOK, CODE, kUcapBatch>(e, lt, tid, smem);
+        run_code<T, S, NC, VCAP, CCOK, CODE, kUcapBatch>(e, lt, tid, smem, more);
         return;
       }
     }
-    dispatch<T, S, NC, VCAP, CCOK, CODE + 1>(v, e, lt, tid, smem);
+    dispatch<T, S, NC, VCAP, CCOK, CODE + 1>(v, e, lt, tid, smem, more);
   }
 }
 
-// Persistent kernel.  Tile slots are taken in static round-robin order
-// (block b runs slots b, b + G, b + 2G, ..., each renumbered to a tile so
-// that one XCD runs neighbouring tiles), so at any moment the blocks work on
-// neighbouring tiles and sweep memory in order, as a one-tile-per-block
-// launch does, without a shared counter.  CODE >= 0: every entry has that
+// Batch kernel: a work-queue walker over the launch's tile slots (block b
+// takes slots b, b + G, b + 2G, ... for a grid of G blocks; launched with one
+// block per slot by default, or as a persistent grid of resident blocks);
+// each slot is renumbered to a tile so that one XCD runs neighbouring tiles.  CODE >= 0: every entry has that
 // variant (one instance per variant); CODE == kMerged: each entry's variant
 // is dispatched at run time (one instance per element type).
 constexpr int kMerged = -1;
@@ -123,6 +122,7 @@ constexpr int kMerged = -1;
 template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
 __global__ __launch_bounds__(kBlock) void k_batch(const BatchEntry *__restrict__ ents,
                                                   const uint32_t *__restrict__ begins,
+                                                  const uint32_t *__restrict__ hints,
                                                   uint32_t nent, uint32_t total) {
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t tid = threadIdx.x;
@@ -130,17 +130,25 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchEntry *__restrict__
   uint32_t ei = 0;
   uint32_t first = 0;
   uint32_t end = (nent > 1) ? __builtin_amdgcn_readfirstlane(begins[1]) : total;
-  for (uint32_t i = blockIdx.x; i < total; i += gridDim.x) {
-    // Blocks are dispatched round-robin over the 8 XCDs, and with a grid
-    // that is a multiple of 8 the tiles i with i % 8 == x all run on XCD x.
-    // Renumber each group of 64 so that those are 8 consecutive tiles:
+  const uint32_t nblocks = gridDim.x * gridDim.y;
+  for (uint32_t i = blockIdx.y * gridDim.x + blockIdx.x; i < total; i += nblocks) {
+    // Blocks are dispatched round-robin over the 8 XCDs, so slots i with
+    // i % 8 == x run on XCD x (for a grid that is a multiple of 8 or covers
+    // every slot).  Renumber each group of 64 so that those are 8
+    // consecutive tiles:
     // neighbouring tiles share cache lines at their edges, and each XCD has
     // its own L2.  (A bijection on [0, total); the last partial group keeps
     // its order.)
     const uint32_t t = i < full64 ? (i & ~63u) | ((i & 7u) << 3) | ((i >> 3) & 7u) : i;
     if (t < first || t >= end) {
-      // Entry of t: the last entry whose first tile is <= t.
+      // Entry of t: the last entry whose first tile is <= t, between the
+      // entries holding tiles 64 (t / 64) and 64 (t / 64 + 1) when hinted.
       uint32_t lo = 0, hi = nent;
+      if (hints) {
+        lo = __builtin_amdgcn_readfirstlane(hints[t >> 6]);
+        hi = __builtin_amdgcn_readfirstlane(hints[(t >> 6) + 1]) + 1;
+        hi = hi < nent ? hi : nent;
+      }
       while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
         if (__builtin_amdgcn_readfirstlane(begins[mid]) <= t) lo = mid;
@@ -151,11 +159,12 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchEntry *__restrict__
       end = (ei + 1 < nent) ? __builtin_amdgcn_readfirstlane(begins[ei + 1]) : total;
     }
     const BatchEntry &e = ents[ei];
+    const bool more = i + nblocks < total;
     if constexpr (CODE == kMerged)
       dispatch<T, S, NC, VCAP, CCOK, 0>(__builtin_amdgcn_readfirstlane(e.variant), e, t - first,
-                                        tid, smem);
+                                        tid, smem, more);
     else
-      run_code<T, S, NC, VCAP, CCOK, CODE, kUcapBatch>(e, t - first, tid, smem);
+      run_code<T, S, NC, VCAP, CCOK, CODE, kUcapBatch>(e, t - first, tid, smem, more);
   }
 }
 

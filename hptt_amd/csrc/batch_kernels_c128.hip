@@ -1,4 +1,4 @@
-// batch_kernels_c128.hip -- every persistent batch kernel instance for complex<double>.
+// batch_kernels_c128.hip -- every batch kernel instance for complex<double>.
 #include "batch_kernels.h"
 
 namespace hpttx {

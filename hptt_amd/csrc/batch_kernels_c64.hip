@@ -1,4 +1,4 @@
-// batch_kernels_c64.hip -- every persistent batch kernel instance for complex<float>.
+// batch_kernels_c64.hip -- every batch kernel instance for complex<float>.
 #include "batch_kernels.h"
 
 namespace hpttx {

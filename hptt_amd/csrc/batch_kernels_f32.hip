@@ -1,4 +1,4 @@
-// batch_kernels_f32.hip -- every persistent batch kernel instance for float.
+// batch_kernels_f32.hip -- every batch kernel instance for float.
 #include "batch_kernels.h"
 
 namespace hpttx {

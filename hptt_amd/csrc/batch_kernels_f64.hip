@@ -1,4 +1,4 @@
-// batch_kernels_f64.hip -- every persistent batch kernel instance for double.
+// batch_kernels_f64.hip -- every batch kernel instance for double.
 #include "batch_kernels.h"
 
 namespace hpttx {

@@ -104,11 +104,6 @@ int Server::start(const ServerConfig &cfg, std::string &err) {
       return e == hipErrorOutOfMemory ? kENoMem : kEHip;
     }
   }
-  // Look up the persistent kernels' occupancy for every LDS size a launch
-  // can use now, not on the request path.
-  for (int dt = kF32; dt <= kC128; ++dt)
-    for (uint32_t v = 0; v <= 128; ++v)
-      for (uint32_t lds = 4096; lds <= 24576; lds += 4096) (void)batch_grid(dt, v, lds, 1u << 30);
   stop_ = false;
   dispatcher_ = std::thread([this] { dispatch_loop(); });
   completer_ = std::thread([this] { complete_loop(); });
@@ -335,8 +330,8 @@ void Server::dispatch_loop() {
     for (size_t gi = 0; gi < groups.size() && he == hipSuccess; ++gi) {
       const Group &g = groups[gi];
       he = launch_batch_group(g.dtype, g.variant, slot->d_ent + g.first, slot->d_begins + g.first,
-                              g.count, g.total, g.lds, batch_grid(g.dtype, g.variant, g.lds, g.total),
-                              slot->stream);
+                              nullptr, g.count, g.total, g.lds,
+                              batch_grid(g.dtype, g.variant, g.lds, g.total), slot->stream);
     }
     if (he == hipSuccess) he = hipEventRecord(slot->done, slot->stream);
     lk.lock();

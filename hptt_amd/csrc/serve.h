@@ -6,7 +6,7 @@
 // them in arrival order while the bytes in flight stay within a budget taken
 // from free HBM and an output arena (for requests that let the server
 // allocate B) has room, packs every admitted request into one launch of the
-// persistent batch kernel per element type, and issues it on the next free
+// batch kernel per element type, and issues it on the next free
 // stream slot -- so requests that arrive while others run are admitted as
 // slots and memory free up, and their launches overlap on the device.  A
 // completer thread polls each slot's event and stamps completion times.

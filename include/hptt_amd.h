@@ -150,8 +150,9 @@ void hpttx_plan_destroy(hpttx_plan *plan);
 
 /*
  * Batched transpositions: many independent plans whose block tiles form one
- * queue, walked by persistent kernels (one launch per element type and
- * kernel variant, or per element type when HPTTX_BATCH_MODE=merged), captured
+ * queue, walked by work-queue kernels (one launch per element type and
+ * kernel variant, or per element type when HPTTX_BATCH_MODE=merged; one block
+ * per tile, or a persistent grid with HPTTX_BATCH_GRID=persistent), captured
  * once in a hipGraph and replayed by hpttx_batch_exec.  No counterpart in the
  * reference, whose only way to run several transpositions is one
  * plan->exec() after another
@@ -196,7 +197,7 @@ typedef struct hpttx_batch_info {
   uint32_t groups;             /* (element type, variant) groups = launches per exec */
   uint64_t algorithmic_bytes;  /* sum over items of N*elem*(2 + update) */
   uint64_t tiles;              /* block tiles over all groups */
-  uint32_t grid[4];            /* persistent blocks of the first four groups */
+  uint32_t grid[4];            /* blocks of the first four groups' launches */
   uint32_t lds_bytes[4];
 } hpttx_batch_info;
 int hpttx_batch_get_info(const hpttx_batch *batch, hpttx_batch_info *info);

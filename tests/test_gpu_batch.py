@@ -1,6 +1,6 @@
-"""Batched mode on the GPU: one persistent work-queue launch per element type,
-replayed from a hipGraph, against the oracle and against sequential plans --
-bit for bit."""
+"""Batched mode on the GPU: work-queue launches per element type and kernel
+variant, replayed from a hipGraph, against the oracle and against sequential
+plans -- bit for bit."""
 import numpy as np
 import pytest
 
@@ -113,6 +113,30 @@ def test_direct_launch_equals_graph(device):
     torch.cuda.synchronize()
     batch = _batch(items, bufs)
     batch.use_graph(False)
+    batch.exec()
+    batch.sync()
+    for it, b in zip(items, bufs):
+        assert oracle.first_mismatch(host(b["dB"]), _expected(it, b)) == -1
+
+
+def test_persistent_grid_mode(device):
+    """HPTTX_BATCH_GRID=persistent launches the same kernels on a grid of
+    resident blocks that walk the tile queue; results match the oracle and
+    the grid is smaller than the tile count."""
+    import os
+
+    import torch
+    rng = np.random.default_rng(13)
+    items = _random_items(rng, 120, [0, 1, 2, 3])
+    bufs = _materialize(items, 1400)
+    torch.cuda.synchronize()
+    os.environ["HPTTX_BATCH_GRID"] = "persistent"
+    try:
+        batch = _batch(items, bufs)
+    finally:
+        del os.environ["HPTTX_BATCH_GRID"]
+    info = batch.info()
+    assert info.tiles > sum(info.grid[:min(4, info.groups)]) or info.groups > 4
     batch.exec()
     batch.sync()
     for it, b in zip(items, bufs):
