@@ -49,7 +49,8 @@ class PlanInfo(ctypes.Structure):
         ("grid", ctypes.c_uint32), ("lds_bytes", ctypes.c_uint32),
         ("tile_r", ctypes.c_uint32), ("tile_c", ctypes.c_uint32),
         ("tile_o", ctypes.c_uint32), ("vec_r", ctypes.c_uint32),
-        ("vec_w", ctypes.c_uint32)]
+        ("vec_w", ctypes.c_uint32), ("unaligned_r", ctypes.c_uint32),
+        ("unaligned_w", ctypes.c_uint32)]
 
 
 def lib() -> ctypes.CDLL:

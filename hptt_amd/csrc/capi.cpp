@@ -225,11 +225,14 @@ int hpttx_plan_get_info(const hpttx_plan *p, hpttx_plan_info *info) {
     info->tile_o = p->cur.tile.TO;
     info->vec_r = p->cur.tile.vr;
     info->vec_w = p->cur.tile.vw;
+    info->unaligned_r = p->cur.tile.ua_r;
+    info->unaligned_w = p->cur.tile.ua_w;
   } else {
     info->tile_r = p->cur.rows.SEG;
     info->tile_c = 1;
     info->tile_o = p->cur.rows.RB;
     info->vec_r = info->vec_w = p->cur.rows.v;
+    info->unaligned_r = info->unaligned_w = p->cur.rows.ua;
   }
   return HPTTX_OK;
 }

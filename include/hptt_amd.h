@@ -142,6 +142,8 @@ typedef struct hpttx_plan_info {
   uint32_t grid;            /* blocks per launch */
   uint32_t lds_bytes;       /* LDS per block */
   uint32_t tile_r, tile_c, tile_o, vec_r, vec_w;
+  uint32_t unaligned_r, unaligned_w; /* 1: that side's runs use vectors at any
+                                        element offset (partial last vectors) */
 } hpttx_plan_info;
 int hpttx_plan_get_info(const hpttx_plan *plan, hpttx_plan_info *info);
 

@@ -81,12 +81,33 @@ def test_padded_and_misaligned(device, dt):
         _vs_oracle(dt, size, perm, 1.5, 0.25, offset=1, seed=k)
 
 
+@pytest.mark.parametrize("dt", shapes.DTYPES)
+def test_unaligned_runs(device, dt):
+    """Odd extents: runs start at any element offset and end in partial
+    vectors (the unaligned-vector kernels), with and without beta, on aligned
+    and one-element-offset pointers."""
+    rng = np.random.default_rng(31)
+    odd = [[7, 5, 3], [3, 9, 11, 2], [5, 7, 6, 3, 2], [13, 3], [3, 5, 7, 2, 3, 2],
+           [31, 33], [127, 65, 33], [31, 33, 35, 7], [1001, 999], [30, 30, 30, 30]]
+    seen_unaligned = False
+    for k, size in enumerate(odd):
+        perm = [int(v) for v in rng.permutation(len(size))]
+        if perm == list(range(len(size))):
+            perm = perm[::-1]
+        for be in (0.0, 4.2):
+            for off in (0, 1):
+                info = _vs_oracle(dt, size, perm, 2.3, be, offset=off, seed=10 * k + off)
+                seen_unaligned |= bool(info.unaligned_r or info.unaligned_w)
+    assert seen_unaligned or np.dtype(dt).itemsize == 16
+
+
 def test_benchmark_workloads_full_size(device):
     """The benchmark workloads at their full sizes, against the oracle."""
     info = _vs_oracle(np.float32, *shapes.CONFIG1, 1.0, 0.0)
     assert hptt_amd.PATHS[info.path] == "tile"
     info = _vs_oracle(np.float32, *shapes.CONFIG2, 2.3, 4.2)
     assert info.update == 1 and info.vec_r == 4 and info.vec_w == 4
+    assert info.unaligned_r == 0 and info.unaligned_w == 0
     _vs_oracle(np.float64, *shapes.CONFIG2, 2.3, 4.2)
 
 
