@@ -385,7 +385,9 @@ void Server::complete_loop() {
       else if (e != hipErrorNotReady) done.push_back(s);   // surfaced at the next sync
     }
     if (done.empty()) {
-      std::this_thread::sleep_for(std::chrono::microseconds(20));
+      // Launches are in flight: keep polling (a timed sleep would add the
+      // kernel timer's slack, tens of microseconds, to every completion).
+      std::this_thread::yield();
       continue;
     }
     const double t = now();
