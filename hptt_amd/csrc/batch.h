@@ -85,20 +85,22 @@ uint32_t batch_grid(int dtype, uint32_t variant, uint32_t lds_bytes, uint32_t to
 // `total`, on `grid` blocks (a 2-D grid beyond 2^20 blocks).  `variant` is
 // a variant code (every entry has it) or batch_group_key(., true) for the
 // instance that dispatches per entry.
-// d_hints: null, or the entry holding tile 64 j for every j plus a final
-// nent - 1 (batch_hints), which narrows each block's entry search.
+// d_ent_of: null (each block binary-searches d_begins for its tile's entry)
+// or the entry of every tile (batch_entry_of_tile: one load per block).
 hipError_t launch_batch_group(int dtype, uint32_t variant, const BatchEntry *d_ent,
-                              const uint32_t *d_begins, const uint32_t *d_hints, uint32_t nent,
+                              const uint32_t *d_begins, const uint32_t *d_ent_of, uint32_t nent,
                               uint32_t total, uint32_t lds_bytes, uint32_t grid, hipStream_t st);
-// Search hints for entries whose first tiles are `begins` (total tiles).
-std::vector<uint32_t> batch_hints(const std::vector<uint32_t> &begins, uint32_t total);
+// Entry index of each of `total` tiles for entries whose first tiles are
+// `begins`.
+std::vector<uint32_t> batch_entry_of_tile(const std::vector<uint32_t> &begins, uint32_t total);
 // Kernel key of an entry's group: its variant, or the per-type instance
 // that dispatches variants at run time when `merged`.
 uint32_t batch_group_key(uint32_t variant, bool merged);
 // Grouping: one launch per element type (merged) or one per (element type,
-// variant) (split).  The environment variable HPTTX_BATCH_MODE ("merged" /
-// "split") overrides `fallback`; batches default to split, the server to
-// merged (fewer launches per slot: lower latency).
+// variant) (split), each further split by LDS class.  The environment
+// variable HPTTX_BATCH_MODE ("merged" / "split") overrides `fallback`;
+// batches and the server default to split (measured faster for both: each
+// instance holds only its own variant's registers).
 bool batch_merged_default(bool fallback);
 
 // The entries of one launch: one element type, and one variant unless merged.
@@ -111,7 +113,7 @@ struct BatchGroup {
   uint32_t grid = 0;
   BatchEntry *d_ent = nullptr;      // device table
   uint32_t *d_begins = nullptr;     // device: first tile of each entry
-  uint32_t *d_hints = nullptr;      // device: batch_hints of the group
+  uint32_t *d_ent_of = nullptr;     // device: entry of each tile
   std::vector<BatchEntry> h_ent;    // host copy
   std::vector<uint32_t> order;      // entry k <- user item order[k]
   uint64_t bytes = 0;               // algorithmic bytes

@@ -155,27 +155,23 @@ uint32_t batch_grid(int dtype, uint32_t variant, uint32_t lds_bytes, uint32_t to
   return std::min<uint32_t>(per_gpu, total_tiles);
 }
 
-std::vector<uint32_t> batch_hints(const std::vector<uint32_t> &begins, uint32_t total) {
-  const size_t nh = ((size_t)total + 63) / 64 + 1;
-  std::vector<uint32_t> h(nh, 0);
+std::vector<uint32_t> batch_entry_of_tile(const std::vector<uint32_t> &begins, uint32_t total) {
+  std::vector<uint32_t> t(total, 0);
   const uint32_t nent = (uint32_t)begins.size();
-  uint32_t e = 0;
-  for (size_t j = 0; j + 1 < nh; ++j) {
-    const uint64_t t = (uint64_t)j * 64;
-    while (e + 1 < nent && begins[e + 1] <= t) ++e;
-    h[j] = e;
+  for (uint32_t e = 0; e < nent; ++e) {
+    const uint32_t end = e + 1 < nent ? begins[e + 1] : total;
+    for (uint32_t k = begins[e]; k < end; ++k) t[k] = e;
   }
-  h[nh - 1] = nent ? nent - 1 : 0;
-  return h;
+  return t;
 }
 
 hipError_t launch_batch_group(int dtype, uint32_t variant, const BatchEntry *d_ent,
-                              const uint32_t *d_begins, const uint32_t *d_hints, uint32_t nent,
+                              const uint32_t *d_begins, const uint32_t *d_ent_of, uint32_t nent,
                               uint32_t total, uint32_t lds_bytes, uint32_t grid, hipStream_t st) {
   if (!total || !nent) return hipSuccess;
   const void *k = batch_kernel(dtype, variant);
   if (!k) return hipErrorInvalidValue;
-  void *args[] = {(void *)&d_ent, (void *)&d_begins, (void *)&d_hints, (void *)&nent,
+  void *args[] = {(void *)&d_ent, (void *)&d_begins, (void *)&d_ent_of, (void *)&nent,
                   (void *)&total};
   // 2-D grid: x * 256 threads must stay below 2^32 work items per dimension.
   const uint32_t gx = grid < kGridX ? grid : kGridX;
@@ -271,16 +267,16 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
     g.total_tiles = (uint32_t)tiles;
     g.lds_bytes = batch_lds_bytes(g.lds_bytes);
     g.grid = batch_grid(g.dtype, g.variant, g.lds_bytes, g.total_tiles);
-    const std::vector<uint32_t> hints = batch_hints(begins, g.total_tiles);
+    const std::vector<uint32_t> ent_of = batch_entry_of_tile(begins, g.total_tiles);
     he = hipMalloc(&g.d_ent, sizeof(BatchEntry) * g.nent);
     if (he == hipSuccess) he = hipMalloc(&g.d_begins, sizeof(uint32_t) * g.nent);
-    if (he == hipSuccess) he = hipMalloc(&g.d_hints, sizeof(uint32_t) * hints.size());
+    if (he == hipSuccess) he = hipMalloc(&g.d_ent_of, sizeof(uint32_t) * ent_of.size());
     if (he == hipSuccess)
       he = hipMemcpy(g.d_ent, g.h_ent.data(), sizeof(BatchEntry) * g.nent, hipMemcpyHostToDevice);
     if (he == hipSuccess)
       he = hipMemcpy(g.d_begins, begins.data(), sizeof(uint32_t) * g.nent, hipMemcpyHostToDevice);
     if (he == hipSuccess)
-      he = hipMemcpy(g.d_hints, hints.data(), sizeof(uint32_t) * hints.size(),
+      he = hipMemcpy(g.d_ent_of, ent_of.data(), sizeof(uint32_t) * ent_of.size(),
                      hipMemcpyHostToDevice);
     b.bytes += g.bytes;
     b.groups.push_back(std::move(g));
@@ -333,7 +329,7 @@ hipError_t batch_enqueue(Batch &b, hipStream_t st) {
   for (size_t i = 0; i < b.groups.size() && e == hipSuccess; ++i) {
     const BatchGroup &g = b.groups[i];
     hipStream_t ls = b.lane[i] == 0 ? st : b.side[b.lane[i] - 1];
-    e = launch_batch_group(g.dtype, g.variant, g.d_ent, g.d_begins, g.d_hints, g.nent,
+    e = launch_batch_group(g.dtype, g.variant, g.d_ent, g.d_begins, g.d_ent_of, g.nent,
                            g.total_tiles,
                            g.lds_bytes, g.grid, ls);
   }
@@ -371,8 +367,8 @@ void batch_free(Batch &b) {
   for (BatchGroup &g : b.groups) {
     if (g.d_ent) (void)hipFree(g.d_ent);
     if (g.d_begins) (void)hipFree(g.d_begins);
-    if (g.d_hints) (void)hipFree(g.d_hints);
-    g.d_hints = nullptr;
+    if (g.d_ent_of) (void)hipFree(g.d_ent_of);
+    g.d_ent_of = nullptr;
     g.d_ent = nullptr;
     g.d_begins = nullptr;
   }

@@ -31,8 +31,8 @@ namespace bk {
 template <typename T, typename S, int NC, int VR, int VW, bool UR, bool UW, bool UPD, bool CC,
           bool SH, bool NT, int UCAP>
 __device__ __forceinline__ void tile_one(const TileDesc &d, uint32_t lt, uint32_t tid,
-                                         const T *A, T *B, unsigned char *smem,
-                                         const Coef<S> &co, bool more) {
+                                         const HPTTX_GLOBAL T *A, HPTTX_GLOBAL T *B,
+                                         unsigned char *smem, const Coef<S> &co, bool more) {
   T *lds = reinterpret_cast<T *>(smem);
   const TileTabs tb = tile_tabs(d, smem, sizeof(T));
   const TileCoord tc = tile_coord(d, lt);
@@ -46,8 +46,8 @@ __device__ __forceinline__ void tile_one(const TileDesc &d, uint32_t lt, uint32_
 
 template <typename T, typename S, int NC, int V, bool UA, bool UPD, bool CC, bool NT, int UCAP>
 __device__ __forceinline__ void rows_one(const RowDesc &d, uint32_t lt, uint32_t tid,
-                                         const T *A, T *B, unsigned char *smem,
-                                         const Coef<S> &co, bool more) {
+                                         const HPTTX_GLOBAL T *A, HPTTX_GLOBAL T *B,
+                                         unsigned char *smem, const Coef<S> &co, bool more) {
   int64_t *tabA = reinterpret_cast<int64_t *>(smem);
   int64_t *tabB = tabA + d.RB;
   const RowCoord rc = row_coord(d, lt);
@@ -84,8 +84,9 @@ __device__ __forceinline__ void run_code(const BatchEntry &e, uint32_t lt, uint3
   constexpr bool cc = CODE & kVarCc;
   constexpr bool nt = CODE & kVarNt;
   const Coef<S> co{(S)e.ar, (S)e.ai, (S)e.br, (S)e.bi};
-  const T *A = static_cast<const T *>(e.A);
-  T *B = static_cast<T *>(e.B);
+  // the entry's pointers are device memory: address them as global
+  const HPTTX_GLOBAL T *A = (const HPTTX_GLOBAL T *)e.A;
+  HPTTX_GLOBAL T *B = (HPTTX_GLOBAL T *)e.B;
   if constexpr (rows) {
     // exact_w on rows: unaligned vectors
     constexpr int v = (exact_r || exact_w) ? VCAP : 1;
@@ -122,41 +123,38 @@ constexpr int kMerged = -1;
 template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
 __global__ __launch_bounds__(kBlock) void k_batch(const BatchEntry *__restrict__ ents,
                                                   const uint32_t *__restrict__ begins,
-                                                  const uint32_t *__restrict__ hints,
+                                                  const uint32_t *__restrict__ ent_of,
                                                   uint32_t nent, uint32_t total) {
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t tid = threadIdx.x;
   const uint32_t full64 = total & ~63u;
   uint32_t ei = 0;
-  uint32_t first = 0;
-  uint32_t end = (nent > 1) ? __builtin_amdgcn_readfirstlane(begins[1]) : total;
+  uint32_t first = 0, end = 0;   // tiles of entry ei: [first, end); empty forces a lookup
   const uint32_t nblocks = gridDim.x * gridDim.y;
   for (uint32_t i = blockIdx.y * gridDim.x + blockIdx.x; i < total; i += nblocks) {
     // Blocks are dispatched round-robin over the 8 XCDs, so slots i with
     // i % 8 == x run on XCD x (for a grid that is a multiple of 8 or covers
     // every slot).  Renumber each group of 64 so that those are 8
-    // consecutive tiles:
-    // neighbouring tiles share cache lines at their edges, and each XCD has
-    // its own L2.  (A bijection on [0, total); the last partial group keeps
-    // its order.)
+    // consecutive tiles: neighbouring tiles share cache lines at their
+    // edges, and each XCD has its own L2.  (A bijection on [0, total); the
+    // last partial group keeps its order.)
     const uint32_t t = i < full64 ? (i & ~63u) | ((i & 7u) << 3) | ((i >> 3) & 7u) : i;
     if (t < first || t >= end) {
-      // Entry of t: the last entry whose first tile is <= t, between the
-      // entries holding tiles 64 (t / 64) and 64 (t / 64 + 1) when hinted.
-      uint32_t lo = 0, hi = nent;
-      if (hints) {
-        lo = __builtin_amdgcn_readfirstlane(hints[t >> 6]);
-        hi = __builtin_amdgcn_readfirstlane(hints[(t >> 6) + 1]) + 1;
-        hi = hi < nent ? hi : nent;
+      if (ent_of) {
+        // the entry of every tile, tabulated at batch creation
+        ei = __builtin_amdgcn_readfirstlane(ent_of[t]);
+      } else {
+        // the last entry whose first tile is <= t
+        uint32_t lo = 0, hi = nent;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (__builtin_amdgcn_readfirstlane(begins[mid]) <= t) lo = mid;
+          else hi = mid;
+        }
+        ei = lo;
       }
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (__builtin_amdgcn_readfirstlane(begins[mid]) <= t) lo = mid;
-        else hi = mid;
-      }
-      ei = lo;
-      first = __builtin_amdgcn_readfirstlane(begins[ei]);
-      end = (ei + 1 < nent) ? __builtin_amdgcn_readfirstlane(begins[ei + 1]) : total;
+      first = __builtin_amdgcn_readfirstlane(ents[ei].tile_begin);
+      end = first + __builtin_amdgcn_readfirstlane(ents[ei].ntiles);
     }
     const BatchEntry &e = ents[ei];
     const bool more = i + nblocks < total;

@@ -32,6 +32,17 @@
 #pragma clang fp contract(off)
 #endif
 
+// A and B are addressed through pointers in the global address space on the
+// device, so the backend emits global_load / global_store (a generic pointer
+// read from a table gives flat accesses, which also count against the LDS
+// wait counter and so serialise LDS waits behind global memory).  Plain
+// pointers on the host.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HPTTX_GLOBAL __attribute__((address_space(1)))
+#else
+#define HPTTX_GLOBAL
+#endif
+
 // Access hook: the host emulator used by the tests defines it to bounds-check
 // and count every global and LDS access; kernels compile it away.
 #ifndef HPTTX_ACCESS
@@ -204,6 +215,36 @@ HPTTX_HD void vstore(T *p, const T (&in)[V]) {
 #endif
 }
 
+// The same for A and B in global memory (UA: only element-aligned).
+template <typename T, int V, bool NT = false, bool UA = false>
+HPTTX_HD void gload(const HPTTX_GLOBAL T *p, T (&out)[V]) {
+  typedef typename RawVec<sizeof(T) * V, UA>::type R;
+  HPTTX_ACCESS(p, sizeof(R), false);
+#if defined(__HIP_DEVICE_COMPILE__)
+  R r = NT ? __builtin_nontemporal_load(reinterpret_cast<const HPTTX_GLOBAL R *>(p))
+           : *reinterpret_cast<const HPTTX_GLOBAL R *>(p);
+#else
+  R r = *reinterpret_cast<const R *>(p);
+#endif
+  __builtin_memcpy(&out[0], &r, sizeof(R));
+}
+
+template <typename T, int V, bool NT, bool UA = false>
+HPTTX_HD void gstore(HPTTX_GLOBAL T *p, const T (&in)[V]) {
+  typedef typename RawVec<sizeof(T) * V, UA>::type R;
+  R r;
+  HPTTX_ACCESS(p, sizeof(R), true);
+  __builtin_memcpy(&r, &in[0], sizeof(R));
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (NT)
+    __builtin_nontemporal_store(r, reinterpret_cast<HPTTX_GLOBAL R *>(p));
+  else
+    *reinterpret_cast<HPTTX_GLOBAL R *>(p) = r;
+#else
+  *reinterpret_cast<R *>(p) = r;
+#endif
+}
+
 // r = fl(a*x) or fl(fl(b*y) + fl(a*x)) per component; complex coefficients
 // use ax = (fl(fl(ar*xr) - fl(ai*xi)), fl(fl(ar*xi) + fl(ai*xr))), same for by.
 template <typename S, int NC, bool UPD, bool CC>
@@ -327,23 +368,23 @@ constexpr int unroll_for() {
 // Load the n <= V elements of a partial vector one by one (the rest of `out`
 // is left zero and never stored to B).
 template <typename T, int V, bool NT>
-HPTTX_HD void load_part(const T *p, uint32_t n, T (&out)[V]) {
+HPTTX_HD void load_part(const HPTTX_GLOBAL T *p, uint32_t n, T (&out)[V]) {
 #pragma unroll
   for (int j = 0; j < V; ++j) {
     T one[1];
     __builtin_memset(&one[0], 0, sizeof(T));
-    if ((uint32_t)j < n) vload<T, 1, NT>(p + j, one);
+    if ((uint32_t)j < n) gload<T, 1, NT>(p + j, one);
     out[j] = one[0];
   }
 }
 
 template <typename T, int V, bool NT>
-HPTTX_HD void store_part(T *p, uint32_t n, const T (&in)[V]) {
+HPTTX_HD void store_part(HPTTX_GLOBAL T *p, uint32_t n, const T (&in)[V]) {
 #pragma unroll
   for (int j = 0; j < V; ++j) {
     if ((uint32_t)j < n) {
       const T one[1] = {in[j]};
-      vstore<T, 1, NT>(p + j, one);
+      gstore<T, 1, NT>(p + j, one);
     }
   }
 }
@@ -354,7 +395,8 @@ HPTTX_HD void store_part(T *p, uint32_t n, const T (&in)[V]) {
 // element); the LDS side stays vector-aligned.
 template <typename T, int VR, bool NT, int UCAP, bool UA = false>
 HPTTX_HD void tile_read(const TileDesc &d, const TileCoord &tc, uint32_t tid,
-                        const T *HPTTX_RESTRICT A, T *lds, const TileTabs &tb) {
+                        const HPTTX_GLOBAL T *HPTTX_RESTRICT A, T *lds,
+                        const TileTabs &tb) {
   constexpr int U = unroll_for<T, VR, UCAP>();
   const uint32_t nvr = d.nvr;
   const uint32_t items = nvr * d.TC * tc.olen;
@@ -374,9 +416,9 @@ HPTTX_HD void tile_read(const TileDesc &d, const TileCoord &tc, uint32_t tid,
       ok[u] = (k < items) && (c < tc.clen) && (v * VR < run);
       lidx[u] = row * d.lds_row + v * VR;
       if (ok[u]) {
-        const T *src = A + abase + v * VR + tb.AC[c] + tb.AO[o];
+        const HPTTX_GLOBAL T *src = A + abase + v * VR + tb.AC[c] + tb.AO[o];
         if (!UA || v * VR + VR <= run)
-          vload<T, VR, NT, UA>(src, buf[u]);
+          gload<T, VR, NT, UA>(src, buf[u]);
         else
           load_part<T, VR, NT>(src, run - v * VR, buf[u]);
       }
@@ -430,7 +472,7 @@ HPTTX_HD WriteAddr write_addr(const TileDesc &d, const TileCoord &tc, uint32_t k
 
 template <typename T, int VW, bool UPD, bool NT, int UCAP, bool UA = false>
 HPTTX_HD void tile_write_setup(const TileDesc &d, const TileCoord &tc, uint32_t k0,
-                               const T *HPTTX_RESTRICT B, const TileTabs &tb,
+                               const HPTTX_GLOBAL T *HPTTX_RESTRICT B, const TileTabs &tb,
                                WriteItems<T, VW, UCAP> &w) {
   constexpr int U = WriteItems<T, VW, UCAP>::U;
   w.k0 = k0;
@@ -440,7 +482,7 @@ HPTTX_HD void tile_write_setup(const TileDesc &d, const TileCoord &tc, uint32_t 
     const WriteAddr a = write_addr(d, tc, k0 + (uint32_t)u * kBlock, VW, tb);
     if (a.ok) {
       if (!UA || a.n == VW)
-        vload<T, VW, NT, UA>(B + a.boff, w.y[u]);
+        gload<T, VW, NT, UA>(B + a.boff, w.y[u]);
       else
         load_part<T, VW, NT>(B + a.boff, a.n, w.y[u]);
     }
@@ -449,7 +491,8 @@ HPTTX_HD void tile_write_setup(const TileDesc &d, const TileCoord &tc, uint32_t 
 
 template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT, int UCAP,
           bool UA = false>
-HPTTX_HD void tile_write_finish(const TileDesc &d, const TileCoord &tc, T *HPTTX_RESTRICT B,
+HPTTX_HD void tile_write_finish(const TileDesc &d, const TileCoord &tc,
+                                HPTTX_GLOBAL T *HPTTX_RESTRICT B,
                                 const T *lds, const TileTabs &tb,
                                 const WriteItems<T, VW, UCAP> &w, const Coef<S> &k) {
   constexpr int U = WriteItems<T, VW, UCAP>::U;
@@ -481,7 +524,7 @@ HPTTX_HD void tile_write_finish(const TileDesc &d, const TileCoord &tc, T *HPTTX
       __builtin_memcpy(&out[j], &re, sizeof(T));
     }
     if (!UA || a.n == VW)
-      vstore<T, VW, NT, UA>(B + a.boff, out);
+      gstore<T, VW, NT, UA>(B + a.boff, out);
     else
       store_part<T, VW, NT>(B + a.boff, a.n, out);
   }
@@ -491,7 +534,7 @@ HPTTX_HD void tile_write_finish(const TileDesc &d, const TileCoord &tc, T *HPTTX
 template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT, int UCAP,
           bool UA = false>
 HPTTX_HD void tile_write(const TileDesc &d, const TileCoord &tc, uint32_t tid,
-                         T *HPTTX_RESTRICT B, const T *lds, const TileTabs &tb,
+                         HPTTX_GLOBAL T *HPTTX_RESTRICT B, const T *lds, const TileTabs &tb,
                          const Coef<S> &k) {
   constexpr int U = WriteItems<T, VW, UCAP>::U;
   const uint32_t items = d.nvw * d.TR * tc.olen;
@@ -536,7 +579,8 @@ HPTTX_HD void row_tables(const RowDesc &d, const RowCoord &rc, uint32_t tid,
 template <typename T, typename S, int NC, int V, bool UPD, bool CC, bool NT, int UCAP,
           bool UA = false>
 HPTTX_HD void row_body(const RowDesc &d, const RowCoord &rc, uint32_t tid,
-                       const T *HPTTX_RESTRICT A, T *HPTTX_RESTRICT B,
+                       const HPTTX_GLOBAL T *HPTTX_RESTRICT A,
+                       HPTTX_GLOBAL T *HPTTX_RESTRICT B,
                        const int64_t *tabA, const int64_t *tabB, const Coef<S> &k) {
   constexpr int U = unroll_for<T, V, UCAP>();
   const uint32_t nvs = d.nvs;
@@ -554,11 +598,11 @@ HPTTX_HD void row_body(const RowDesc &d, const RowCoord &rc, uint32_t tid,
       n[u] = ok ? umin32(V, rc.slen - v * V) : 0;
       if (ok) {
         const int64_t e = (int64_t)rc.seg0 + (int64_t)v * V;
-        const T *pa = A + tabA[rr] + e * d.ra;
-        const T *pb = B + tabB[rr] + e * d.rb;
+        const HPTTX_GLOBAL T *pa = A + tabA[rr] + e * d.ra;
+        const HPTTX_GLOBAL T *pb = B + tabB[rr] + e * d.rb;
         if (!UA || n[u] == V) {
-          vload<T, V, NT, UA>(pa, x[u]);
-          if (UPD) vload<T, V, NT, UA>(pb, y[u]);
+          gload<T, V, NT, UA>(pa, x[u]);
+          if (UPD) gload<T, V, NT, UA>(pb, y[u]);
         } else {
           load_part<T, V, NT>(pa, n[u], x[u]);
           if (UPD) load_part<T, V, NT>(pb, n[u], y[u]);
@@ -580,9 +624,9 @@ HPTTX_HD void row_body(const RowDesc &d, const RowCoord &rc, uint32_t tid,
         Elem<S, NC> re = axpby<S, NC, UPD, CC>(xe, ye, k);
         __builtin_memcpy(&out[j], &re, sizeof(T));
       }
-      T *pb = B + tabB[rr] + e * d.rb;
+      HPTTX_GLOBAL T *pb = B + tabB[rr] + e * d.rb;
       if (!UA || n[u] == V)
-        vstore<T, V, NT, UA>(pb, out);
+        gstore<T, V, NT, UA>(pb, out);
       else
         store_part<T, V, NT>(pb, n[u], out);
     }

@@ -39,9 +39,10 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileDesc d, const T *__restrict
   const TileCoord tc = tile_coord(d, t);
   tile_tables(d, tc, tid, tb);
   __syncthreads();
-  tile_read<T, VR, NT, kUcapPlan, UR>(d, tc, tid, A, lds, tb);
+  tile_read<T, VR, NT, kUcapPlan, UR>(d, tc, tid, (const HPTTX_GLOBAL T *)A, lds, tb);
   __syncthreads();
-  tile_write<T, S, NC, VW, UPD, CC, SH, NT, kUcapPlan, UW>(d, tc, tid, B, lds, tb, co);
+  tile_write<T, S, NC, VW, UPD, CC, SH, NT, kUcapPlan, UW>(d, tc, tid, (HPTTX_GLOBAL T *)B, lds,
+                                                           tb, co);
 }
 
 template <typename T, typename S, int NC, int V, bool UA, bool UPD, bool CC, bool NT>
@@ -56,7 +57,8 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowDesc d, const T *__restrict_
   const RowCoord rc = row_coord(d, t);
   row_tables(d, rc, tid, tabA, tabB);
   __syncthreads();
-  row_body<T, S, NC, V, UPD, CC, NT, kUcapPlan, UA>(d, rc, tid, A, B, tabA, tabB, co);
+  row_body<T, S, NC, V, UPD, CC, NT, kUcapPlan, UA>(d, rc, tid, (const HPTTX_GLOBAL T *)A,
+                                                   (HPTTX_GLOBAL T *)B, tabA, tabB, co);
 }
 
 namespace {

@@ -284,30 +284,33 @@ void Server::dispatch_loop() {
     stats_.launches++;
     lk.unlock();
 
-    // One launch per element type (or per (type, variant)); within one,
-    // largest first.
-    const bool merged = batch_merged_default(true);
+    // One launch per (element type, kernel variant, LDS class), as for
+    // batches (or per element type and LDS class with HPTTX_BATCH_MODE=
+    // merged); within one, largest first.
+    const bool merged = batch_merged_default(false);
     for (Request *r : batch) {
       BatchEntry probe;
       make_entry(r->plan, r->spec, probe);
       r->variant = batch_group_key(probe.variant, merged);
+      r->lds_class = (probe.lds_bytes + 8191u) / 8192u;
     }
     std::stable_sort(batch.begin(), batch.end(), [](const Request *a, const Request *b) {
       if (a->spec.dtype != b->spec.dtype) return a->spec.dtype < b->spec.dtype;
       if (a->variant != b->variant) return a->variant < b->variant;
+      if (a->lds_class != b->lds_class) return a->lds_class < b->lds_class;
       return a->plan.p.N > b->plan.p.N;
     });
     struct Group {
       int dtype;
-      uint32_t variant, first, count, total, lds;
+      uint32_t variant, lds_class, first, count, total, lds;
     };
     std::vector<Group> groups;
     hipError_t he = hipSuccess;
     for (uint32_t i = 0; i < batch.size(); ++i) {
       Request *r = batch[i];
       if (groups.empty() || groups.back().dtype != r->spec.dtype ||
-          groups.back().variant != r->variant)
-        groups.push_back(Group{r->spec.dtype, r->variant, i, 0, 0, 0});
+          groups.back().variant != r->variant || groups.back().lds_class != r->lds_class)
+        groups.push_back(Group{r->spec.dtype, r->variant, r->lds_class, i, 0, 0, 0});
       Group &g = groups.back();
       BatchEntry &e = slot->h_ent[i];
       make_entry(r->plan, r->spec, e);

@@ -53,6 +53,7 @@ struct Request {
   int state = kQueued;
   int status = 0;
   uint32_t variant = 0;          // batch-kernel variant of its plan
+  uint32_t lds_class = 0;        // LDS need in 8 KiB classes (launch grouping)
   double t_submit = 0, t_launch = 0, t_done = 0;
 };
 

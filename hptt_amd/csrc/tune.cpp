@@ -81,7 +81,7 @@ std::vector<TuneParams> tune_candidates(const Problem &p) {
   const TuneParams d = default_tune(p);
   c.push_back(d);
   const uint32_t e = p.esize;
-  const uint32_t runs[] = {64 / e, 128 / e, 256 / e, 512 / e};
+  const uint32_t runs[] = {64 / e, 128 / e, 256 / e, 512 / e, 1024 / e};
   const uint32_t blocks[] = {2048 / e, 4096 / e, 8192 / e, 16384 / e};
   const uint32_t wides[] = {128, 256, 1024};
   for (uint32_t rr : runs)
