@@ -91,6 +91,32 @@ int build_problem(int dtype, uint32_t rank, const uint32_t *in_size,
     }
     p.d[m++] = tmp[i];
   }
+  // Kernels index a fused dim with 32 bits: split any dim of 2^31 elements
+  // or more into (n / a, a), a its smallest divisor that leaves both parts
+  // below 2^31, searched among the 2^20 candidates above n / 2^31 (an 8 GiB
+  // fp32 tensor fuses to one such dim under an identity perm; 288 GB of HBM
+  // holds tensors far larger).
+  for (int i = 0; i < m; ++i) {
+    const uint64_t n = p.d[i].n;
+    if (n < (1ull << 31)) continue;
+    if (m >= kMaxRank) {
+      err = "a fused dim of 2^31 or more elements in a rank-6 problem";
+      return kEUnsupported;
+    }
+    uint64_t a = (n >> 31) + 1;
+    const uint64_t a_end = std::min<uint64_t>(1ull << 31, a + (1ull << 20));   // bounded search
+    while (a < a_end && n % a) ++a;
+    if (a >= a_end || n / a >= (1ull << 31)) {
+      err = "a fused dim cannot be split below 2^31";
+      return kEUnsupported;
+    }
+    for (int k = m; k > i + 1; --k) p.d[k] = p.d[k - 1];
+    const FusedDim whole = p.d[i];
+    p.d[i] = FusedDim{n / a, whole.sa, whole.sb};
+    p.d[i + 1] = FusedDim{a, whole.sa * (int64_t)(n / a), whole.sb * (int64_t)(n / a)};
+    ++m;
+    --i;   // re-check the first part (already below 2^31)
+  }
   p.fr = m;
   for (int i = 0; i < m; ++i) p.border[i] = i;
   std::sort(p.border, p.border + m, [&](int x, int y) { return p.d[x].sb < p.d[y].sb; });

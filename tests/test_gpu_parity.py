@@ -174,3 +174,35 @@ def test_plan_interface(device):
     exp = np.full(n, 7.0)
     oracle.transpose(A1, exp, size, perm, 2.0, 1.0)
     assert oracle.first_mismatch(host(dB3), exp) == -1
+
+
+def test_tensors_beyond_2_31_elements(device):
+    """2^31 + 2^16 fp32 elements (8.6 GB per tensor): an identity copy whose
+    one fused dim is split for 32-bit indexing, and a transpose round trip
+    (perm then its inverse) with 64-bit offsets, compared bit for bit."""
+    import torch
+    size = [65536, 32769]
+    n = size[0] * size[1]
+    assert n > 2**31
+    A = torch.empty(n, device="cuda", dtype=torch.float32).uniform_(-1, 1)
+    B = torch.empty_like(A)
+    hptt_amd.create_plan_checked(A, B, size, [0, 1], 1.0, 0.0).exec()
+    torch.cuda.synchronize()
+    assert torch.equal(A.view(torch.int32), B.view(torch.int32))
+    p1 = hptt_amd.create_plan_checked(A, B, size, [1, 0], 1.0, 0.0)
+    C = torch.empty_like(A)
+    p2 = hptt_amd.create_plan_checked(B, C, size[::-1], [1, 0], 1.0, 0.0)
+    p1.exec()
+    p2.exec()
+    torch.cuda.synchronize()
+    assert torch.equal(A.view(torch.int32), C.view(torch.int32))
+    # the transposed layout: A is {65536, 32769} column-major, so a row-major
+    # view (32769, 65536) holds A(i0, i1) at [i1, i0]; B is {32769, 65536}
+    # with B(i1, i0) = A(i0, i1) at row-major [i0, i1]
+    Av = A.view(size[1], size[0])
+    Bv = B.view(size[0], size[1])
+    for i0, i1 in [(0, 0), (65535, 32768), (12345, 23456), (1, 32767)]:
+        assert Bv[i0, i1].item() == Av[i1, i0].item()
+    del A, B, C
+    torch.cuda.empty_cache()
+

@@ -149,3 +149,16 @@ def test_fusion_and_validation():
     assert E.plan(np.float32, [4, 0], [1, 0])[0] == 1
     assert E.plan(np.float32, [2] * 7, list(range(7)))[0] == 4
     assert E.plan(np.float32, [4, 4], [1, 0], [3, 4])[0] == 1
+
+
+def test_huge_fused_dims_are_split():
+    """A fused dim of 2^31 or more elements (the kernels index with 32 bits)
+    is split into two below 2^31; one that cannot be split is refused."""
+    rc, err, d = E.plan(np.float32, [65536, 131072], [0, 1])     # 2^33, identity
+    assert rc == 0, err
+    assert "fused dims (A order, size:strideA:strideB): 1073741824:1:1 8:1073741824:1073741824" in d
+    rc, err, d = E.plan(np.float32, [2**16, 2**16, 3], [1, 0, 2])
+    assert rc == 0 and "path: tile" in d
+    rc, err, d = E.plan(np.float32, [5, 4294967291], [0, 1])     # prime > 2^31
+    assert rc == 4 and "split" in err
+
