@@ -141,6 +141,12 @@ struct TileDesc {
   // whole number of vectors), 1 = unaligned (vectors at any element offset,
   // a run's last partial vector done element by element).
   uint32_t ua_r, ua_w;
+  // LDS chunk swizzle (rows of a power-of-two number of 16-byte chunks, no
+  // padding): chunk k of LDS row `row` is stored at k ^ ((row >> swz_shift)
+  // & swz_mask).  The read pass's vector stores stay conflict-free and the
+  // write pass's column gathers (consecutive lanes on rows VW apart) spread
+  // over the banks; swz_mask = 0 is the plain padded layout.
+  uint32_t swz_mask, swz_shift;
   FastDiv div_nR, div_nC, div_nvr, div_TC, div_nvw, div_TR, div_VS;
   DimList R, C, O;
 };
@@ -414,7 +420,7 @@ HPTTX_HD void tile_read(const TileDesc &d, const TileCoord &tc, uint32_t tid,
       const uint32_t o = fdiv(row, d.div_TC);
       const uint32_t c = row - o * d.TC;
       ok[u] = (k < items) && (c < tc.clen) && (v * VR < run);
-      lidx[u] = row * d.lds_row + v * VR;
+      lidx[u] = row * d.lds_row + (v ^ ((row >> d.swz_shift) & d.swz_mask)) * VR;
       if (ok[u]) {
         const HPTTX_GLOBAL T *src = A + abase + v * VR + tb.AC[c] + tb.AO[o];
         if (!UA || v * VR + VR <= run)
@@ -446,7 +452,8 @@ struct WriteItems {
 
 struct WriteAddr {
   int64_t boff;     // B element offset of the vector
-  uint32_t lbase;   // LDS offset of (o, c = 0, r, s = 0)
+  uint32_t lrow0;   // LDS row of (o, c = 0)
+  uint32_t x0;      // element of (r, s = 0) within an LDS row
   uint32_t q0;      // first element of the vector within the run
   uint32_t n;       // elements of the vector inside the run (vw unless partial)
   bool ok;
@@ -465,7 +472,8 @@ HPTTX_HD WriteAddr write_addr(const TileDesc &d, const TileCoord &tc, uint32_t k
   w.ok = (kk < items) && (r < tc.rlen) && (v * vw < run);
   w.q0 = v * vw;
   w.n = w.ok ? umin32(vw, run - w.q0) : 0;
-  w.lbase = o * d.TC * d.lds_row + r * d.VS;
+  w.lrow0 = o * d.TC;
+  w.x0 = r * d.VS;
   w.boff = w.ok ? (int64_t)tc.c0 * d.VS + v * vw + tb.BR[r] + tb.BO[o] : 0;
   return w;
 }
@@ -511,7 +519,13 @@ HPTTX_HD void tile_write_finish(const TileDesc &d, const TileCoord &tc,
         c = fdiv(q, d.div_VS);
         s = q - c * d.VS;
       }
-      const T *src = lds + a.lbase + c * d.lds_row + s;
+      // LDS element (row lrow0 + c, x0 + s), through the chunk swizzle
+      // (chunks of 16 / sizeof(T) elements, the read pass's vectors)
+      constexpr uint32_t kChunk = sizeof(T) >= 16 ? 1 : 16 / sizeof(T);
+      const uint32_t lrow = a.lrow0 + c, xc = a.x0 + s;
+      const uint32_t xs = ((xc / kChunk) ^ ((lrow >> d.swz_shift) & d.swz_mask)) * kChunk +
+                          xc % kChunk;
+      const T *src = lds + lrow * d.lds_row + xs;
       HPTTX_ACCESS(src, sizeof(T), false);
       x[j] = *src;
     }

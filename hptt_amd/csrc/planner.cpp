@@ -419,12 +419,23 @@ void finalize_vectors(const Problem &p, LaunchPlan &lp, uint32_t vr, uint32_t vw
     t.nvw = (t.TC * t.VS + cap - 1) / cap;
     t.div_nvr = make_fastdiv(t.nvr);
     t.div_nvw = make_fastdiv(t.nvw);
-    // LDS rows: a multiple of the read vector, padded by one vector so that
-    // the write pass's column gathers spread over banks (odd for scalars).
+    // LDS rows: a multiple of the read vector.  Rows of a power-of-two
+    // number (>= 4) of vectors use the chunk swizzle (measured 1.4% faster
+    // than padding on the 6-D fp32 workload); others are padded by one
+    // vector (one element for scalars, to an odd stride) so that the write
+    // pass's column gathers spread over banks.
     uint32_t row = t.TR * t.VS;
     row = (row + cap - 1) / cap * cap;
     t.lds_row = row + cap;
     if (cap == 1 && (row % 2) == 0) t.lds_row = row + 1;
+    t.swz_mask = 0;
+    t.swz_shift = 0;
+    const uint32_t chunks = row / cap;
+    if (cap > 1 && t.VS == 1 && chunks >= 4 && (chunks & (chunks - 1)) == 0 && t.TC % cap == 0) {
+      t.lds_row = row;
+      t.swz_mask = chunks - 1;
+      while ((1u << t.swz_shift) < cap) ++t.swz_shift;
+    }
     lp.lds_bytes = tile_lds_bytes(t, p.esize);
     lp.grid = t.ntiles;
   } else {
@@ -504,7 +515,8 @@ std::string describe(const Problem &p, const LaunchPlan &lp) {
     os << "\ntile: VS=" << t.VS << " R=" << t.VR << "/" << t.TR << " C=" << t.VC << "/" << t.TC
        << " O=" << t.VO << "/" << t.TO << " vec(r,w)=" << t.vr << (t.ua_r ? "u" : "") << ","
        << t.vw << (t.ua_w ? "u" : "")
-       << " tiles=" << t.ntiles << " lds=" << lp.lds_bytes << "B";
+       << " tiles=" << t.ntiles << " lds=" << lp.lds_bytes << "B"
+       << (t.swz_mask ? " swizzled" : " padded");
   } else {
     const RowDesc &r = lp.rows;
     os << "\nrows: L=" << r.L << " stride(a,b)=" << r.ra << "," << r.rb << " rows=" << r.nrows

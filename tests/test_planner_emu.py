@@ -122,7 +122,7 @@ def test_emulated_benchmark_shapes_reduced():
 
 def test_plans_for_benchmark_shapes():
     rc, err, d = E.plan(np.float32, *shapes.CONFIG2)
-    assert rc == 0 and "path: tile\n" in d and "vec(r,w)=4,4" in d, d
+    assert rc == 0 and "path: tile\n" in d and "vec(r,w)=4,4" in d and "swizzled" in d, d
     rc, err, d = E.plan(np.float64, *shapes.CONFIG2)
     assert rc == 0 and "vec(r,w)=2,2" in d, d
     rc, err, d = E.plan(np.complex128, *shapes.CONFIG3_TILE)
