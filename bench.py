@@ -379,29 +379,34 @@ def time_serving(count=600, rates=(0.5, 1.0, 2.0), inflight_gb=8.0):
     return out
 
 
-def cpu_baseline(budget_s=10.0):
-    """Oracle (port of the reference's OpenMP RefTrans) on the fp32 workload:
+def cpu_baseline(budget_s=10.0, name="6d-f32"):
+    """Oracle (port of the reference's OpenMP RefTrans) on one workload:
     repeated full transposes for about budget_s seconds of CPU time."""
     import oracle
-    w = WORKLOADS["6d-f32"]
+    w = WORKLOADS[name]
     n = int(np.prod(w["size"]))
+    dt = np.dtype(w["dtype"])
     threads = min(os.cpu_count() or 1, 16)
-    A = oracle.uniform(n, np.float32, 1, nthreads=threads)
-    B = oracle.uniform(n, np.float32, 2, nthreads=threads)
+    A = oracle.uniform(n, dt, 1, nthreads=threads)
+    B = oracle.uniform(n, dt, 2, nthreads=threads)
+    cc = isinstance(w["alpha"], complex) or isinstance(w["beta"], complex)
+    upd = abs(w["beta"]) >= 1e-16
     times = []
     t_start = time.perf_counter()
     while time.perf_counter() - t_start < budget_s and len(times) < 1000:
         t0 = time.perf_counter()
-        rc = oracle.transpose(A, B, w["size"], w["perm"], w["alpha"], w["beta"], nthreads=threads)
+        rc = oracle.transpose(A, B, w["size"], w["perm"], w["alpha"], w["beta"], nthreads=threads,
+                              complex_coef=cc)
         times.append(time.perf_counter() - t0)
         assert rc == 0
     med = float(np.median(times))
-    nbytes = n * 4 * 3
+    nbytes = n * dt.itemsize * (3 if upd else 2)
     return {"value": round(nbytes / med / 1e9, 2), "unit": "GB/s", "cores": threads,
             "kind": "port", "best_gbps": round(nbytes / min(times) / 1e9, 2),
-            "sample": "full 6-D fp32 workload (151M elements, alpha=2.3, beta=4.2) repeated "
-                      "for %.1f s: %d runs, median; oracle OpenMP loop (reference RefTrans "
-                      "order), %d threads" % (time.perf_counter() - t_start, len(times), threads)}
+            "sample": "%s (%s) repeated for %.1f s: %d runs, median; oracle OpenMP loop "
+                      "(reference RefTrans order), %d threads"
+                      % (name, describe_workload(name), time.perf_counter() - t_start,
+                         len(times), threads)}
 
 
 def main():
@@ -468,6 +473,12 @@ def main():
         log("bench: CPU baseline (oracle, OpenMP)")
         cpu = cpu_baseline()
         log("bench: CPU %.2f GB/s on %d threads" % (cpu["value"], cpu["cores"]))
+        if not args.quick:
+            # the same port on the other single-transpose workloads, briefly
+            for name in ("3d-f32-copy", "4d-c128-tile", "6d-f64"):
+                c2 = cpu_baseline(2.0, name)
+                secondary.setdefault("cpu_baseline", {})[name] = c2
+                log("bench: CPU %s %.2f GB/s" % (name, c2["value"]))
     gbps = main_res["gbps"]
     line = {
         "metric": METRIC,
