@@ -154,7 +154,7 @@ def time_workload(name, steps, warmup, tune=0.0):
 def time_batched(dtype_name, reps=5, seq_reps=3):
     """The batched workload: 4096 tensors (seeded manifest, rank 2..6, dims
     log-uniform in [2,128], <= 4 Mi elements each, random perms, 25% beta=0)
-    as one Batch (persistent kernel, graph replay) against one plan->exec()
+    as one Batch (work-queue launches, graph replay) against one plan->exec()
     per tensor in sequence on the same stream; identical bits required."""
     import torch
 

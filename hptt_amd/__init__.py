@@ -333,9 +333,9 @@ def make_batch_item(in_data, out_data, in_size, perm, alpha, beta, dtype=None,
 
 
 class Batch:
-    """Many transpositions executed by persistent launches over one tile
-    queue (one launch per element type and kernel variant, forked over side
-    streams), replayed from a captured hipGraph (hpttx_batch_* in
+    """Many transpositions executed by work-queue launches over one tile
+    queue (one launch per element type, kernel variant and LDS class, forked
+    over side streams), replayed from a captured hipGraph (hpttx_batch_* in
     include/hptt_amd.h).  items: a sequence of BatchItem."""
 
     def __init__(self, items, tuning_timeout: float = 0.0, stream=None, keep=()):

@@ -1,5 +1,5 @@
 """Serving mode on the GPU: requests admitted as they arrive, batched into
-persistent launches on several streams; every result must equal, bit for bit,
+work-queue launches on several streams; every result must equal, bit for bit,
 the same transposition run alone through a plain plan and the oracle."""
 import threading
 import time
