@@ -81,3 +81,36 @@ def test_cpp_dropin_header_compiles(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                         str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+# Where the reference's public headers are in this build container (absent on
+# machines that only run the library: the test then skips).
+REFERENCE_INC = "/root/reference/inc"
+
+# The mangled factory names the reference's front end looks up in its plugin
+# (src/hptt.cc:71-84).
+REFERENCE_FACTORIES = [
+    "_Z24create_trans_plan_impl_sPKfPfjRKSt6vectorIjSaIjEES6_ffjdS6_S6_",
+    "_Z24create_trans_plan_impl_dPKdPdjRKSt6vectorIjSaIjEES6_ddjdS6_S6_",
+    "_Z24create_trans_plan_impl_cPKCfPS_jRKSt6vectorIjSaIjEES7_ffjdS7_S7_",
+    "_Z24create_trans_plan_impl_zPKCdPS_jRKSt6vectorIjSaIjEES7_ddjdS7_S7_",
+]
+
+
+@pytest.mark.skipif(not os.path.isdir(REFERENCE_INC), reason="reference headers not present")
+def test_reference_plugin_shim_exports_the_factories(tmp_path):
+    """tests/cpp/plugin_shim.cpp, built against the reference's own public
+    headers and linked to the library, exports exactly the four factories the
+    reference's front end dlsyms, so the unchanged front end can load it."""
+    so = tmp_path / "libhptt_avx2.so"
+    r = subprocess.run(["g++", "-std=c++17", "-shared", "-fPIC", "-Wall", "-I", REFERENCE_INC,
+                        "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cpp", "plugin_shim.cpp"),
+                        "-L", os.path.join(ROOT, "hptt_amd"), "-lhptt_amd", "-o", str(so)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    nm = subprocess.run(["nm", "-D", "--defined-only", str(so)], capture_output=True, text=True)
+    assert nm.returncode == 0, nm.stderr
+    exported = {line.split()[-1] for line in nm.stdout.splitlines() if line.strip()}
+    for name in REFERENCE_FACTORIES:
+        assert name in exported, name
