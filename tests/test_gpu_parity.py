@@ -206,3 +206,24 @@ def test_tensors_beyond_2_31_elements(device):
     del A, B, C
     torch.cuda.empty_cache()
 
+
+
+def test_tuning_cache_file(device, tmp_path):
+    """tuning_timeout != 0 records the chosen tile parameters in the file
+    named by HPTTX_TUNE_CACHE (key: arch, dtype, update flag, fused problem;
+    value: run_r run_c block wide stream), and the tuned plan is exact."""
+    import os
+    path = tmp_path / "tune.txt"
+    os.environ["HPTTX_TUNE_CACHE"] = str(path)
+    try:
+        size, perm = [48, 20, 36], [1, 2, 0]
+        info = _vs_oracle(np.float32, size, perm, 1.5, 0.5, tt=1.0, seed=77)
+    finally:
+        del os.environ["HPTTX_TUNE_CACHE"]
+    assert info.update == 1
+    lines = path.read_text().splitlines()
+    assert lines, "no tuning record written"
+    key, value = lines[-1].split("\t")
+    # {48, 20, 36} with perm {1, 2, 0} fuses A's dims 1 and 2 (contiguous in both)
+    assert key == "gfx950|0|1|2|48:1:720|720:48:1", key
+    assert len(value.split()) == 5
