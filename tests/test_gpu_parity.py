@@ -227,3 +227,48 @@ def test_tuning_cache_file(device, tmp_path):
     # {48, 20, 36} with perm {1, 2, 0} fuses A's dims 1 and 2 (contiguous in both)
     assert key == "gfx950|0|1|2|48:1:720|720:48:1", key
     assert len(value.split()) == 5
+
+
+def test_concurrent_exec_from_host_threads(device):
+    """Plans are immutable after creation: several host threads exec their own
+    plans, and one shared plan, each on its own stream, at the same time; every
+    result is exact (the C ABI keeps no per-exec host state)."""
+    import threading
+
+    import torch
+    size, perm = [40, 36, 50], [2, 0, 1]
+    n = int(np.prod(size))
+    A = oracle.uniform(n, np.float32, 5)
+    exp = np.zeros(n, np.float32)
+    assert oracle.transpose(A, exp, size, perm, 1.0, 0.0) == 0
+    dA = dev(A)
+    outs = [torch.zeros(n, dtype=torch.float32, device="cuda") for _ in range(8)]
+    streams = [torch.cuda.Stream() for _ in range(8)]
+    torch.cuda.synchronize()
+    # threads 0 and 1 share one plan (same idempotent output, alpha=1,
+    # beta=0), threads 2..7 each own one
+    shared = hptt_amd.create_plan_checked(dA, outs[0], size, perm, 1.0, 0.0)
+    own = {k: hptt_amd.create_plan_checked(dA, outs[k], size, perm, 1.0, 0.0, stream=streams[k])
+           for k in range(2, 8)}
+    errors = []
+
+    def worker(k):
+        try:
+            for _ in range(20):
+                if k < 2:
+                    shared.exec_on(streams[k])
+                else:
+                    own[k].exec()
+            streams[k].synchronize()
+        except Exception as e:  # noqa: BLE001 -- reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(8)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    torch.cuda.synchronize()
+    assert not errors, errors
+    for k in [0] + list(range(2, 8)):
+        assert oracle.first_mismatch(host(outs[k]), exp) == -1, k
