@@ -49,8 +49,9 @@ def _compile(src: str, build_dir: str = BUILD, defines=()) -> str:
 
 
 def build(verbose: bool = False, defines=(), out: str = LIB) -> str:
-    """Compile every source and link `out`.  `defines` (NAME=VALUE) select
-    compile-time policies, e.g. HPTTX_NT_LOAD=0 for ordinary loads."""
+    """Compile every source and link `out`.  `defines` (NAME or NAME=VALUE)
+    are passed to every compile; objects of such builds go to their own
+    _build_<defines> folder."""
     build_dir = BUILD if not defines else BUILD + "_" + "_".join(
         d.replace("=", "") for d in defines)
     os.makedirs(build_dir, exist_ok=True)

@@ -37,9 +37,10 @@ int fail(int code, const std::string &msg) {
 
 bool update_real(double beta) { return std::fabs(beta) >= 1e-16; }
 
-// Vector widths for the current pointers: the shape's widths, dropped to 1
-// when a pointer is not aligned to them (the kernels are instantiated for
-// the full 16-byte vector and for scalars).
+// Run modes for the current pointers: a side is exact when the shape allows
+// full 16-byte vectors and the pointer is 16-byte aligned, and unaligned
+// (full-width vectors at element offsets, partial last vectors) otherwise;
+// see finalize_vectors.
 void refresh_vectors(hpttx_plan *p) {
   const uint32_t esz = p->prob.esize;
   const uint32_t cap = 16 / esz;
