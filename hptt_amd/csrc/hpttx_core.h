@@ -1,9 +1,10 @@
 // hpttx_core.h -- problem descriptors and the per-block bodies of the
 // transposition kernels.
 //
-// Every body here is written once as a host+device function taking an explicit
-// thread id, so the HIP kernels (kernels.hip, batch.hip) run exactly this code
-// on gfx950 between __syncthreads() barriers, and the test-only emulator runs
+// Every body here is written once as a host+device function taking an
+// explicit thread id, so the HIP kernels (kernels.hip, batch_kernels.h) run
+// exactly this code on gfx950 between __syncthreads() barriers, and the
+// test-only emulator runs
 // the same code on the host thread by thread to check every index against the
 // buffer extents before a kernel ever touches a GPU.
 //
