@@ -4,9 +4,8 @@
 // Every body here is written once as a host+device function taking an
 // explicit thread id, so the HIP kernels (kernels.hip, batch_kernels.h) run
 // exactly this code on gfx950 between __syncthreads() barriers, and the
-// test-only emulator runs
-// the same code on the host thread by thread to check every index against the
-// buffer extents before a kernel ever touches a GPU.
+// test-only emulator runs the same code on the host thread by thread to check
+// every index against the buffer extents before a kernel ever touches a GPU.
 //
 // Operation (reference README.md:6-12, inc/hptt/hptt.h:18-22):
 //   B[j] = alpha*A[i] (+ beta*B[j]),  j_d = i_{perm[d]}, column-major A and B.
