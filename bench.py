@@ -151,7 +151,7 @@ def time_workload(name, steps, warmup, tune=0.0):
     return res
 
 
-def time_batched(dtype_name, reps=5, seq_reps=3):
+def time_batched(dtype_name, reps=5, seq_reps=3, tune=0.0):
     """The batched workload: 4096 tensors (seeded manifest, rank 2..6, dims
     log-uniform in [2,128], <= 4 Mi elements each, random perms, 25% beta=0)
     as one Batch (work-queue launches, graph replay) against one plan->exec()
@@ -182,7 +182,7 @@ def time_batched(dtype_name, reps=5, seq_reps=3):
         plans.append(hptt_amd.create_plan_checked(a, b, m["in_size"], m["perm"], m["alpha"],
                                                   m["beta"], stream=stream))
     t_build = time.perf_counter()
-    batch = hptt_amd.Batch(items, stream=stream)
+    batch = hptt_amd.Batch(items, stream=stream, tuning_timeout=tune)
     t_build = time.perf_counter() - t_build
     info = batch.info()
     exec_fn = hptt_amd.lib().hpttx_plan_exec
@@ -234,6 +234,7 @@ def time_batched(dtype_name, reps=5, seq_reps=3):
            "sequential_gbps": round(info.algorithmic_bytes / (seq * 1e-3) / 1e9, 1),
            "algorithmic_bytes": info.algorithmic_bytes, "tiles": info.tiles,
            "grid": list(info.grid)[:info.groups], "build_s": round(t_build, 3),
+           "tuning_timeout": tune, "groups": info.groups,
            "identical_to_sequential": same}
     del batch, plans, A, B, B0, Bseq
     torch.cuda.empty_cache()

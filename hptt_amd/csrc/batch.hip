@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -187,6 +188,159 @@ bool batch_merged_default(bool fallback) {
   return fallback;
 }
 
+namespace {
+
+// Scratch a class tuning run may use (A and B copies together).
+constexpr uint64_t kTuneScratchBytes = 256ull << 20;
+
+uint64_t round256(uint64_t x) { return (x + 255) / 256 * 256; }
+
+// Batch-context tuning of one plan class: K copies of the class's largest
+// member, on scratch buffers, run as one batch launch per candidate tile
+// shape, and the fastest shape wins.  A lone member would be timed on its
+// latency (favouring small tiles); a batch is bound by throughput.
+int tune_class_batch(const PlannedItem &rep, const BatchItemSpec &spec, double budget_s,
+                     hipStream_t st, TuneParams &out, std::string &err) {
+  out = TuneParams{};   // the model's pick unless a candidate beats it
+  const uint64_t es = rep.p.esize;
+  const uint64_t a_bytes = round256(rep.p.spanA * es), b_bytes = round256(rep.p.spanB * es);
+  const uint64_t K = std::max<uint64_t>(1, std::min<uint64_t>(64, kTuneScratchBytes / (a_bytes + b_bytes)));
+  char *A = nullptr, *B = nullptr;
+  if (hipMalloc(&A, K * a_bytes) != hipSuccess || hipMalloc(&B, K * b_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    if (A) (void)hipFree(A);
+    return kOk;   // no room for scratch: keep the model pick
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  BatchEntry *d_ent = nullptr;
+  uint32_t *d_begins = nullptr, *d_ent_of = nullptr;
+  size_t ent_of_cap = 0;
+  int rc = kOk;
+  if (hipMemsetAsync(A, 0, K * a_bytes, st) != hipSuccess ||
+      hipMemsetAsync(B, 0, K * b_bytes, st) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+      hipMalloc(&d_ent, sizeof(BatchEntry) * K) != hipSuccess ||
+      hipMalloc(&d_begins, sizeof(uint32_t) * K) != hipSuccess) {
+    err = "batch tuning: HIP setup failed";
+    rc = kEHip;
+  }
+  struct Cand {
+    TuneParams tp;
+    std::vector<BatchEntry> ent;
+    std::vector<uint32_t> begins;
+    uint32_t total = 0, lds = 0, key = 0;
+  };
+  // Build a candidate's K entries (tile shape from tp, pointers into scratch).
+  auto build = [&](const TuneParams &tp, Cand &c) -> bool {
+    BatchItemSpec s = spec;
+    s.A = A;
+    s.B = B;
+    PlannedItem q;
+    std::string e2;
+    if (plan_item(s, tp, q, e2) != kOk) return false;
+    c.tp = tp;
+    c.ent.assign(K, BatchEntry{});
+    c.begins.assign(K, 0);
+    uint64_t total = 0;
+    for (uint64_t k = 0; k < K; ++k) {
+      s.A = A + k * a_bytes;
+      s.B = B + k * b_bytes;
+      make_entry(q, s, c.ent[k]);
+      c.ent[k].tile_begin = (uint32_t)total;
+      c.begins[k] = (uint32_t)total;
+      total += c.ent[k].ntiles;
+    }
+    if (total > 0xFFFFFFFFull) return false;
+    c.total = (uint32_t)total;
+    c.lds = batch_lds_bytes(c.ent[0].lds_bytes);
+    c.key = batch_group_key(c.ent[0].variant, false);
+    return true;
+  };
+  // Upload a candidate's tables and time `reps` launches (ms per launch).
+  auto time_cand = [&](const Cand &c, int reps, float &ms) -> bool {
+    const std::vector<uint32_t> ent_of = batch_entry_of_tile(c.begins, c.total);
+    if (ent_of.size() > ent_of_cap) {
+      if (d_ent_of) (void)hipFree(d_ent_of);
+      d_ent_of = nullptr;
+      if (hipMalloc(&d_ent_of, sizeof(uint32_t) * ent_of.size()) != hipSuccess) return false;
+      ent_of_cap = ent_of.size();
+    }
+    if (hipMemcpyAsync(d_ent, c.ent.data(), sizeof(BatchEntry) * K, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(d_begins, c.begins.data(), sizeof(uint32_t) * K, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(d_ent_of, ent_of.data(), sizeof(uint32_t) * ent_of.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+      return false;
+    const uint32_t grid = batch_grid(spec.dtype, c.key, c.lds, c.total);
+    if (launch_batch_group(spec.dtype, c.key, d_ent, d_begins, d_ent_of, (uint32_t)K, c.total, c.lds, grid, st) != hipSuccess)
+      return false;
+    (void)hipEventRecord(e0, st);
+    for (int i = 0; i < reps; ++i)
+      (void)launch_batch_group(spec.dtype, c.key, d_ent, d_begins, d_ent_of, (uint32_t)K, c.total, c.lds, grid, st);
+    (void)hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess) return false;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    ms /= reps;
+    return true;
+  };
+  const auto t_start = std::chrono::steady_clock::now();
+  std::vector<std::pair<float, TuneParams>> scored;
+  std::vector<LaunchPlan> seen;
+  if (rc == kOk) {
+    for (const TuneParams &tp : tune_candidates(rep.p)) {
+      if (budget_s > 0 &&
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count() > budget_s)
+        break;
+      // plan once to skip candidates whose launch shape was already timed
+      PlannedItem q;
+      std::string e2;
+      BatchItemSpec s = spec;
+      s.A = A;
+      s.B = B;
+      if (plan_item(s, tp, q, e2) != kOk) continue;
+      bool dup = false;
+      for (const LaunchPlan &l : seen) dup = dup || same_launch(l, q.lp);
+      if (dup) continue;
+      seen.push_back(q.lp);
+      Cand c;
+      float ms = 0;
+      if (!build(tp, c)) continue;
+      if (!time_cand(c, 3, ms)) {
+        err = "batch tuning: launch failed";
+        rc = kEHip;
+        break;
+      }
+      scored.push_back({ms, tp});
+    }
+  }
+  if (rc == kOk && !scored.empty()) {
+    std::sort(scored.begin(), scored.end(),
+              [](const std::pair<float, TuneParams> &x, const std::pair<float, TuneParams> &y) {
+                return x.first < y.first;
+              });
+    float best = 1e30f;
+    for (size_t i = 0; i < scored.size() && i < 3; ++i) {
+      Cand c;
+      float ms = 0;
+      if (!build(scored[i].second, c) || !time_cand(c, 10, ms)) continue;
+      if (ms < best) {
+        best = ms;
+        out = scored[i].second;
+      }
+    }
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipGetLastError();
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (d_ent) (void)hipFree(d_ent);
+  if (d_begins) (void)hipFree(d_begins);
+  if (d_ent_of) (void)hipFree(d_ent_of);
+  (void)hipFree(A);
+  (void)hipFree(B);
+  return rc;
+}
+
+}  // namespace
+
 int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, double tuning_timeout,
                 Batch &b, std::string &err) {
   b = Batch();
@@ -203,28 +357,35 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
     auto it2 = class_rep.find(keys[k]);
     if (it2 == class_rep.end() || pl[it2->second].p.N < pl[k].p.N) class_rep[keys[k]] = k;
   }
-  // Per-class tile shapes: tuned on the class's largest member (scratch
-  // buffers), or taken from the tuning cache.
+  // Per-class tile shapes: tuned in batch context on copies of the class's
+  // largest member (scratch buffers), or taken from the tuning cache (under
+  // a batch key: a lone plan of the same problem may tune differently).
   if (tuning_timeout != 0.0) {
     const double share = tuning_timeout > 0 ? tuning_timeout / std::max<size_t>(1, class_rep.size()) : -1.0;
     std::map<std::string, TuneParams> class_tune;
     for (auto &kv : class_rep) {
       const PlannedItem &rep = pl[kv.second];
+      const std::string key = "batch|" + tune_key(rep.p, rep.update);
       TuneParams tp{};
-      if (!tune_lookup(rep.p, rep.update, tp)) {
-        int rc = autotune(rep.p, items[kv.second].coef, rep.update, rep.cc, share, stream, tp, err);
+      if (!tune_lookup_key(key, tp)) {
+        int rc = tune_class_batch(rep, items[kv.second], share, stream, tp, err);
         if (rc != kOk) return rc;
+        tune_store_key(key, tp);
       }
       class_tune[kv.first] = tp;
     }
+    // A class's tuned shape may not fit every member (a larger tile than
+    // 64 KiB of LDS for a member with other extents): such members keep
+    // their model plan.
     for (size_t k = 0; k < items.size(); ++k) {
-      int rc = plan_item(items[k], class_tune[keys[k]], pl[k], err);
-      if (rc != kOk) { err = "item " + std::to_string(k) + ": " + err; return rc; }
+      PlannedItem tuned;
+      std::string e2;
+      if (plan_item(items[k], class_tune[keys[k]], tuned, e2) == kOk) pl[k] = tuned;
     }
   }
   // Batches: one launch per (element type, variant), forked over streams
   // (measured faster than one dispatching launch per type on the 4096-tensor
-  // mix: 6.7 ms vs 8.2 ms fp32).
+  // mix: 4.93 ms vs 5.53 ms fp32, 8.71 vs 10.18 ms fp64).
   const bool merged = batch_merged_default(false);
   // Groups: (element type, kernel, LDS class).  A launch reserves its
   // largest entry's LDS for every block, so entries are also split by LDS

@@ -49,15 +49,6 @@ void store_locked(const std::string &key, const TuneParams &t) {
            << t.wide_min_bytes << ' ' << t.stream << '\n';
 }
 
-bool same_launch(const LaunchPlan &a, const LaunchPlan &b) {
-  if (a.kind != b.kind) return false;
-  if (a.stream != b.stream) return false;
-  if (a.kind == kTile)
-    return a.tile.TR == b.tile.TR && a.tile.TC == b.tile.TC && a.tile.TO == b.tile.TO &&
-           a.tile.VR == b.tile.VR && a.tile.VC == b.tile.VC && a.tile.VS == b.tile.VS;
-  return a.rows.SEG == b.rows.SEG && a.rows.RB == b.rows.RB && a.rows.L == b.rows.L;
-}
-
 }  // namespace
 
 std::string tune_key(const Problem &p, bool update) {
@@ -74,6 +65,30 @@ bool tune_lookup(const Problem &p, bool update, TuneParams &out) {
   if (it == g_cache.end()) return false;
   out = it->second;
   return true;
+}
+
+bool same_launch(const LaunchPlan &a, const LaunchPlan &b) {
+  if (a.kind != b.kind) return false;
+  if (a.stream != b.stream) return false;
+  if (a.kind == kTile)
+    return a.tile.TR == b.tile.TR && a.tile.TC == b.tile.TC && a.tile.TO == b.tile.TO &&
+           a.tile.VR == b.tile.VR && a.tile.VC == b.tile.VC && a.tile.VS == b.tile.VS;
+  return a.rows.SEG == b.rows.SEG && a.rows.RB == b.rows.RB && a.rows.L == b.rows.L;
+}
+
+bool tune_lookup_key(const std::string &key, TuneParams &out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  load_file_locked();
+  auto it = g_cache.find(key);
+  if (it == g_cache.end()) return false;
+  out = it->second;
+  return true;
+}
+
+void tune_store_key(const std::string &key, const TuneParams &t) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  load_file_locked();
+  store_locked(key, t);
 }
 
 std::vector<TuneParams> tune_candidates(const Problem &p) {

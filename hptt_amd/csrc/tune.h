@@ -34,4 +34,12 @@ int autotune(const Problem &p, const CoefD &coef, bool update, bool cc,
 // The candidate list the tuner walks (model default first).
 std::vector<TuneParams> tune_candidates(const Problem &p);
 
+// The same cache under an explicit key (batch classes are tuned in batch
+// context and cached under their own keys).
+bool tune_lookup_key(const std::string &key, TuneParams &out);
+void tune_store_key(const std::string &key, const TuneParams &t);
+
+// Two launch plans with the same launch shape (tiles or rows, streaming).
+bool same_launch(const LaunchPlan &a, const LaunchPlan &b);
+
 }  // namespace hpttx
