@@ -70,6 +70,8 @@ void make_entry(const PlannedItem &q, const BatchItemSpec &it, BatchEntry &e);
 // LDS per block a batch launch uses for entries needing lds_bytes (a
 // 16-byte multiple; a launch takes the largest of its entries').
 uint32_t batch_lds_bytes(uint32_t lds_bytes);
+// LDS class of an entry (8 KiB steps): split-mode launches group by it.
+uint32_t batch_lds_class(uint32_t lds_bytes);
 // Blocks of one launch.  The kernel is a work-queue walker over the
 // launch's tiles (block b takes slots b, b + grid, ...).  By default it is
 // launched with one block per tile, so the hardware dispatcher hands out the

@@ -116,6 +116,8 @@ void make_entry(const PlannedItem &q, const BatchItemSpec &it, BatchEntry &e) {
 
 uint32_t batch_lds_bytes(uint32_t lds_bytes) { return (lds_bytes + 15u) & ~15u; }
 
+uint32_t batch_lds_class(uint32_t lds_bytes) { return (lds_bytes + 8191u) / 8192u; }
+
 bool batch_persistent() {
   const char *m = std::getenv("HPTTX_BATCH_GRID");
   return m && std::strcmp(m, "persistent") == 0;
@@ -389,14 +391,16 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
   const bool merged = batch_merged_default(false);
   // Groups: (element type, kernel, LDS class).  A launch reserves its
   // largest entry's LDS for every block, so entries are also split by LDS
-  // need (8 KiB classes): small tiles keep their occupancy.
+  // need (8 KiB classes): small tiles keep their occupancy.  Merged mode
+  // keeps one launch per element type (with HPTTX_BATCH_GRID=persistent:
+  // one persistent work-queue launch per type).
   std::map<std::tuple<int, uint32_t, uint32_t>, std::vector<size_t>> by_kernel;
   for (size_t k = 0; k < items.size(); ++k) {
     b.paths[k] = pl[k].lp.path;
     BatchEntry probe;
     make_entry(pl[k], items[k], probe);
     by_kernel[std::make_tuple(items[k].dtype, batch_group_key(probe.variant, merged),
-                              (probe.lds_bytes + 8191u) / 8192u)]
+                              merged ? 0u : batch_lds_class(probe.lds_bytes))]
         .push_back(k);
   }
   hipError_t he = hipSuccess;

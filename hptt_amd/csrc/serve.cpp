@@ -292,7 +292,7 @@ void Server::dispatch_loop() {
       BatchEntry probe;
       make_entry(r->plan, r->spec, probe);
       r->variant = batch_group_key(probe.variant, merged);
-      r->lds_class = (probe.lds_bytes + 8191u) / 8192u;
+      r->lds_class = merged ? 0u : batch_lds_class(probe.lds_bytes);
     }
     std::stable_sort(batch.begin(), batch.end(), [](const Request *a, const Request *b) {
       if (a->spec.dtype != b->spec.dtype) return a->spec.dtype < b->spec.dtype;

@@ -200,3 +200,30 @@ def test_set_data_and_tuned_classes(device):
         if k == 5:
             continue
         assert oracle.first_mismatch(host(b["dB"]), _expected(it, b)) == -1, k
+
+
+def test_single_persistent_launch_per_type(device):
+    """HPTTX_BATCH_MODE=merged with HPTTX_BATCH_GRID=persistent: one persistent
+    work-queue launch per element type (the kernel dispatches each entry's
+    variant), replayed from the graph; results match the oracle."""
+    import os
+
+    import torch
+    rng = np.random.default_rng(17)
+    items = _random_items(rng, 90, [0, 1])
+    bufs = _materialize(items, 1500)
+    torch.cuda.synchronize()
+    os.environ["HPTTX_BATCH_MODE"] = "merged"
+    os.environ["HPTTX_BATCH_GRID"] = "persistent"
+    try:
+        batch = _batch(items, bufs)
+    finally:
+        del os.environ["HPTTX_BATCH_MODE"]
+        del os.environ["HPTTX_BATCH_GRID"]
+    info = batch.info()
+    assert info.groups == len({it["code"] for it in items})
+    assert all(g < info.tiles for g in info.grid[:info.groups])
+    batch.exec()
+    batch.sync()
+    for it, b in zip(items, bufs):
+        assert oracle.first_mismatch(host(b["dB"]), _expected(it, b)) == -1
