@@ -50,7 +50,7 @@ class PlanInfo(ctypes.Structure):
         ("tile_r", ctypes.c_uint32), ("tile_c", ctypes.c_uint32),
         ("tile_o", ctypes.c_uint32), ("vec_r", ctypes.c_uint32),
         ("vec_w", ctypes.c_uint32), ("unaligned_r", ctypes.c_uint32),
-        ("unaligned_w", ctypes.c_uint32)]
+        ("unaligned_w", ctypes.c_uint32), ("threads", ctypes.c_uint32)]
 
 
 def lib() -> ctypes.CDLL:

@@ -220,6 +220,7 @@ int hpttx_plan_get_info(const hpttx_plan *p, hpttx_plan_info *info) {
   info->algorithmic_bytes = p->prob.N * p->prob.esize * (uint64_t)(2 + (p->update ? 1 : 0));
   info->grid = p->cur.grid;
   info->lds_bytes = (uint32_t)p->cur.lds_bytes;
+  info->threads = p->cur.threads;
   if (p->cur.kind == kTile) {
     info->tile_r = p->cur.tile.TR * p->cur.tile.VS;
     info->tile_c = p->cur.tile.TC * p->cur.tile.VS;

@@ -341,19 +341,22 @@ HPTTX_HD TileTabs tile_tabs(const TileDesc &d, void *lds_base, size_t esize) {
   return t;
 }
 
+// NTH: threads per block (every body below takes it; kBlock unless a
+// single-plan launch uses wider blocks).
+template <uint32_t NTH = kBlock>
 HPTTX_HD void tile_tables(const TileDesc &d, const TileCoord &tc, uint32_t tid,
                           const TileTabs &tb) {
-  for (uint32_t i = tid; i < tc.clen; i += kBlock) {
+  for (uint32_t i = tid; i < tc.clen; i += NTH) {
     int64_t a, b;
     dl_offsets(d.C, tc.c0 + i, a, b);
     tb.AC[i] = a;
   }
-  for (uint32_t i = tid; i < tc.rlen; i += kBlock) {
+  for (uint32_t i = tid; i < tc.rlen; i += NTH) {
     int64_t a, b;
     dl_offsets(d.R, tc.r0 + i, a, b);
     tb.BR[i] = b;
   }
-  for (uint32_t i = tid; i < tc.olen; i += kBlock) {
+  for (uint32_t i = tid; i < tc.olen; i += NTH) {
     int64_t a, b;
     dl_offsets(d.O, tc.o0 + i, a, b);
     tb.AO[i] = a;
@@ -399,7 +402,7 @@ HPTTX_HD void store_part(HPTTX_GLOBAL T *p, uint32_t n, const T (&in)[V]) {
 // from A's run r0*VS + [0, rlen*VS) into LDS row `row`.  UA: the run may
 // start at any element offset and end in a partial vector (loaded element by
 // element); the LDS side stays vector-aligned.
-template <typename T, int VR, bool NT, int UCAP, bool UA = false>
+template <typename T, int VR, bool NT, int UCAP, bool UA = false, uint32_t NTH = kBlock>
 HPTTX_HD void tile_read(const TileDesc &d, const TileCoord &tc, uint32_t tid,
                         const HPTTX_GLOBAL T *HPTTX_RESTRICT A, T *lds,
                         const TileTabs &tb) {
@@ -408,13 +411,13 @@ HPTTX_HD void tile_read(const TileDesc &d, const TileCoord &tc, uint32_t tid,
   const uint32_t items = nvr * d.TC * tc.olen;
   const uint32_t run = tc.rlen * d.VS;
   const int64_t abase = (int64_t)tc.r0 * d.VS;
-  for (uint32_t k0 = tid; k0 < items; k0 += kBlock * U) {
+  for (uint32_t k0 = tid; k0 < items; k0 += NTH * U) {
     T buf[U][VR];
     uint32_t lidx[U];
     bool ok[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint32_t k = k0 + (uint32_t)u * kBlock;
+      const uint32_t k = k0 + (uint32_t)u * NTH;
       const uint32_t row = fdiv(k, d.div_nvr);
       const uint32_t v = k - row * nvr;
       const uint32_t o = fdiv(row, d.div_TC);
@@ -447,7 +450,7 @@ template <typename T, int VW, int UCAP>
 struct WriteItems {
   static constexpr int U = unroll_for<T, VW, UCAP>();
   T y[U][VW];       // B's current values (UPDATE only)
-  uint32_t k0;      // first item; item u is k0 + u*kBlock
+  uint32_t k0;      // first item; item u is k0 + u*NTH
 };
 
 struct WriteAddr {
@@ -478,7 +481,8 @@ HPTTX_HD WriteAddr write_addr(const TileDesc &d, const TileCoord &tc, uint32_t k
   return w;
 }
 
-template <typename T, int VW, bool UPD, bool NT, int UCAP, bool UA = false>
+template <typename T, int VW, bool UPD, bool NT, int UCAP, bool UA = false,
+          uint32_t NTH = kBlock>
 HPTTX_HD void tile_write_setup(const TileDesc &d, const TileCoord &tc, uint32_t k0,
                                const HPTTX_GLOBAL T *HPTTX_RESTRICT B, const TileTabs &tb,
                                WriteItems<T, VW, UCAP> &w) {
@@ -487,7 +491,7 @@ HPTTX_HD void tile_write_setup(const TileDesc &d, const TileCoord &tc, uint32_t 
   if (!UPD) return;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const WriteAddr a = write_addr(d, tc, k0 + (uint32_t)u * kBlock, VW, tb);
+    const WriteAddr a = write_addr(d, tc, k0 + (uint32_t)u * NTH, VW, tb);
     if (a.ok) {
       if (!UA || a.n == VW)
         gload<T, VW, NT, UA>(B + a.boff, w.y[u]);
@@ -498,7 +502,7 @@ HPTTX_HD void tile_write_setup(const TileDesc &d, const TileCoord &tc, uint32_t 
 }
 
 template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT, int UCAP,
-          bool UA = false>
+          bool UA = false, uint32_t NTH = kBlock>
 HPTTX_HD void tile_write_finish(const TileDesc &d, const TileCoord &tc,
                                 HPTTX_GLOBAL T *HPTTX_RESTRICT B,
                                 const T *lds, const TileTabs &tb,
@@ -506,7 +510,7 @@ HPTTX_HD void tile_write_finish(const TileDesc &d, const TileCoord &tc,
   constexpr int U = WriteItems<T, VW, UCAP>::U;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const WriteAddr a = write_addr(d, tc, w.k0 + (uint32_t)u * kBlock, VW, tb);
+    const WriteAddr a = write_addr(d, tc, w.k0 + (uint32_t)u * NTH, VW, tb);
     if (!a.ok) continue;
     T x[VW];
 #pragma unroll
@@ -546,16 +550,16 @@ HPTTX_HD void tile_write_finish(const TileDesc &d, const TileCoord &tc,
 
 // The whole write pass.
 template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT, int UCAP,
-          bool UA = false>
+          bool UA = false, uint32_t NTH = kBlock>
 HPTTX_HD void tile_write(const TileDesc &d, const TileCoord &tc, uint32_t tid,
                          HPTTX_GLOBAL T *HPTTX_RESTRICT B, const T *lds, const TileTabs &tb,
                          const Coef<S> &k) {
   constexpr int U = WriteItems<T, VW, UCAP>::U;
   const uint32_t items = d.nvw * d.TR * tc.olen;
-  for (uint32_t k0 = tid; k0 < items; k0 += kBlock * U) {
+  for (uint32_t k0 = tid; k0 < items; k0 += NTH * U) {
     WriteItems<T, VW, UCAP> w;
-    tile_write_setup<T, VW, UPD, NT, UCAP, UA>(d, tc, k0, B, tb, w);
-    tile_write_finish<T, S, NC, VW, UPD, CC, SH, NT, UCAP, UA>(d, tc, B, lds, tb, w, k);
+    tile_write_setup<T, VW, UPD, NT, UCAP, UA, NTH>(d, tc, k0, B, tb, w);
+    tile_write_finish<T, S, NC, VW, UPD, CC, SH, NT, UCAP, UA, NTH>(d, tc, B, lds, tb, w, k);
   }
 }
 

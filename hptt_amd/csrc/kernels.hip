@@ -23,13 +23,13 @@
 
 namespace hpttx {
 
-// One block per tile on a 2-D grid (t = y * gridDim.x + x).  UR / UW:
-// unaligned runs on the read / write side (full-width vectors at any element
-// offset, partial last vectors element by element).
+// One block of NTH threads per tile on a 2-D grid (t = y * gridDim.x + x).
+// UR / UW: unaligned runs on the read / write side (full-width vectors at any
+// element offset, partial last vectors element by element).
 template <typename T, typename S, int NC, int VR, int VW, bool UR, bool UW, bool UPD, bool CC,
-          bool SH, bool NT>
-__global__ __launch_bounds__(kBlock) void k_tile(TileDesc d, const T *__restrict__ A,
-                                                 T *__restrict__ B, Coef<S> co) {
+          bool SH, bool NT, uint32_t NTH>
+__global__ __launch_bounds__(NTH) void k_tile(TileDesc d, const T *__restrict__ A,
+                                              T *__restrict__ B, Coef<S> co) {
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t t = blockIdx.y * gridDim.x + blockIdx.x;
   if (t >= d.ntiles) return;
@@ -37,12 +37,12 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileDesc d, const T *__restrict
   const TileTabs tb = tile_tabs(d, smem, sizeof(T));
   const uint32_t tid = threadIdx.x;
   const TileCoord tc = tile_coord(d, t);
-  tile_tables(d, tc, tid, tb);
+  tile_tables<NTH>(d, tc, tid, tb);
   __syncthreads();
-  tile_read<T, VR, NT, kUcapPlan, UR>(d, tc, tid, (const HPTTX_GLOBAL T *)A, lds, tb);
+  tile_read<T, VR, NT, kUcapPlan, UR, NTH>(d, tc, tid, (const HPTTX_GLOBAL T *)A, lds, tb);
   __syncthreads();
-  tile_write<T, S, NC, VW, UPD, CC, SH, NT, kUcapPlan, UW>(d, tc, tid, (HPTTX_GLOBAL T *)B, lds,
-                                                           tb, co);
+  tile_write<T, S, NC, VW, UPD, CC, SH, NT, kUcapPlan, UW, NTH>(d, tc, tid, (HPTTX_GLOBAL T *)B,
+                                                                lds, tb, co);
 }
 
 template <typename T, typename S, int NC, int V, bool UA, bool UPD, bool CC, bool NT>
@@ -69,9 +69,9 @@ hipError_t launch_typed(const LaunchPlan &lp, const void *A, void *B,
   const T *a = static_cast<const T *>(A);
   T *b = static_cast<T *>(B);
   if (lp.grid == 0) return hipSuccess;
-  // 2-D grid: x * 256 threads must stay below 2^32 work items per dimension.
+  // 2-D grid: x * threads must stay below 2^32 work items per dimension.
   const uint32_t gx = lp.grid < kGridX ? lp.grid : kGridX;
-  dim3 grid(gx, (lp.grid + gx - 1) / gx), block(kBlock);
+  dim3 grid(gx, (lp.grid + gx - 1) / gx), block(kBlock), wide(kWideBlock);
   // Per side: exact vectors, unaligned vectors (VCAP > 1) or scalars (VCAP == 1).
   constexpr bool kUA = VCAP > 1;
   if (lp.kind == kTile) {
@@ -79,8 +79,14 @@ hipError_t launch_typed(const LaunchPlan &lp, const void *A, void *B,
     const bool sh = d.VS > 1;
     const bool ur = d.ua_r != 0, uw = d.ua_w != 0;
 #define HPTTX_TILE(UR_, UW_, SH_)                                                              \
-  hipLaunchKernelGGL((k_tile<T, S, NC, VCAP, VCAP, UR_, UW_, UPD, CC, SH_, NT>), grid, block, \
-                     lp.lds_bytes, st, d, a, b, co)
+  do {                                                                                        \
+    if (lp.threads == kWideBlock)                                                             \
+      hipLaunchKernelGGL((k_tile<T, S, NC, VCAP, VCAP, UR_, UW_, UPD, CC, SH_, NT, kWideBlock>), \
+                         grid, wide, lp.lds_bytes, st, d, a, b, co);                          \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_tile<T, S, NC, VCAP, VCAP, UR_, UW_, UPD, CC, SH_, NT, kBlock>),  \
+                         grid, block, lp.lds_bytes, st, d, a, b, co);                         \
+  } while (0)
     if (sh) {
       if (!ur && !uw) HPTTX_TILE(false, false, true);
       else if (!ur) HPTTX_TILE(false, kUA, true);

@@ -437,6 +437,7 @@ void finalize_vectors(const Problem &p, LaunchPlan &lp, uint32_t vr, uint32_t vw
       while ((1u << t.swz_shift) < cap) ++t.swz_shift;
     }
     lp.lds_bytes = tile_lds_bytes(t, p.esize);
+    lp.threads = lp.lds_bytes > kWideBlockMinLds ? kWideBlock : kBlock;
     lp.grid = t.ntiles;
   } else {
     RowDesc &r = lp.rows;
@@ -446,6 +447,7 @@ void finalize_vectors(const Problem &p, LaunchPlan &lp, uint32_t vr, uint32_t vw
     r.nvs = (r.SEG + r.v - 1) / r.v;
     r.div_nvs = make_fastdiv(r.nvs);
     lp.lds_bytes = row_lds_bytes(r);
+    lp.threads = kBlock;
     lp.grid = r.ntiles;
   }
   (void)vw;
@@ -523,7 +525,7 @@ std::string describe(const Problem &p, const LaunchPlan &lp) {
        << " seg=" << r.SEG << " rows/block=" << r.RB << " vec=" << r.v << (r.ua ? "u" : "")
        << " tiles=" << r.ntiles;
   }
-  os << "\ngrid: " << lp.grid << " x " << kBlock << "  streaming: " << (lp.stream ? "on" : "off")
+  os << "\ngrid: " << lp.grid << " x " << lp.threads << "  streaming: " << (lp.stream ? "on" : "off")
      << "  tune(run_r,run_c,block,wide,stream)=" << lp.tune.run_r << "," << lp.tune.run_c << ","
      << lp.tune.block_elems << "," << lp.tune.wide_min_bytes << "," << lp.tune.stream << "\n";
   return os.str();

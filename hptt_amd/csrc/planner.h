@@ -52,6 +52,12 @@ struct Problem {
 
 enum Kind : int { kTile = 0, kRows = 1 };
 
+// Tiles whose LDS footprint lets at most 4 blocks share a CU (more than
+// 32 KiB of its 160 KiB) run 512-thread blocks, so those 4 blocks still fill
+// all 32 wave slots; smaller tiles keep kBlock threads and more blocks.
+constexpr uint32_t kWideBlock = 512;
+constexpr size_t kWideBlockMinLds = 32768;
+
 struct LaunchPlan {
   Kind kind = kRows;
   TileDesc tile{};
@@ -59,6 +65,7 @@ struct LaunchPlan {
   uint32_t vr_shape = 1, vw_shape = 1;  // widest vectors the strides allow
   uint32_t grid = 0;
   size_t lds_bytes = 0;
+  uint32_t threads = kBlock;   // threads per block (kBlock or kWideBlock)
   bool stream = false;         // non-temporal loads and stores
   TuneParams tune{};
   std::string path;            // "tile", "tile-shared", "wide-copy", "axpby", "gather"

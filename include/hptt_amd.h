@@ -144,6 +144,7 @@ typedef struct hpttx_plan_info {
   uint32_t tile_r, tile_c, tile_o, vec_r, vec_w;
   uint32_t unaligned_r, unaligned_w; /* 1: that side's runs use vectors at any
                                         element offset (partial last vectors) */
+  uint32_t threads;         /* threads per block (256, or 512 for tiles over 32 KiB of LDS) */
 } hpttx_plan_info;
 int hpttx_plan_get_info(const hpttx_plan *plan, hpttx_plan_info *info);
 

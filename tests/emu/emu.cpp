@@ -64,7 +64,7 @@ using namespace hpttx;
 namespace {
 
 template <typename T, typename S, int NC, int VR, int VW, bool UR, bool UW, bool UPD, bool CC,
-          bool SH>
+          bool SH, uint32_t NTH>
 void run_tile(const LaunchPlan &lp, const T *A, T *B, const Coef<S> &k) {
   const TileDesc &d = lp.tile;
   std::vector<uint64_t> smem((lp.lds_bytes + 7) / 8 + 2);
@@ -74,11 +74,15 @@ void run_tile(const LaunchPlan &lp, const T *A, T *B, const Coef<S> &k) {
   const TileTabs tb = tile_tabs(d, smem.data(), sizeof(T));
   for (uint32_t t = 0; t < d.ntiles; ++t) {
     const TileCoord tc = tile_coord(d, t);
-    for (uint32_t tid = 0; tid < kBlock; ++tid) tile_tables(d, tc, tid, tb);
-    for (uint32_t tid = 0; tid < kBlock; ++tid)
-      tile_read<T, VR, false, kUcapPlan, UR>(d, tc, tid, A, lds, tb);
-    for (uint32_t tid = 0; tid < kBlock; ++tid)
-      tile_write<T, S, NC, VW, UPD, CC, SH, false, kUcapPlan, UW>(d, tc, tid, B, lds, tb, k);
+    // poison LDS (NaN bytes) so that a read of a slot the read pass did not
+    // fill shows up in B
+    std::memset(smem.data(), 0xff, lp.lds_bytes);
+    for (uint32_t tid = 0; tid < NTH; ++tid) tile_tables<NTH>(d, tc, tid, tb);
+    for (uint32_t tid = 0; tid < NTH; ++tid)
+      tile_read<T, VR, false, kUcapPlan, UR, NTH>(d, tc, tid, A, lds, tb);
+    for (uint32_t tid = 0; tid < NTH; ++tid)
+      tile_write<T, S, NC, VW, UPD, CC, SH, false, kUcapPlan, UW, NTH>(d, tc, tid, B, lds, tb,
+                                                                      k);
     if (!emu::g_fault.empty()) return;
   }
 }
@@ -107,7 +111,13 @@ void run_typed(const LaunchPlan &lp, const void *A, void *B, const Coef<S> &k) {
   if (lp.kind == kTile) {
     const TileDesc &d = lp.tile;
     const bool sh = d.VS > 1, ur = d.ua_r != 0, uw = d.ua_w != 0;
-#define EMU_TILE(UR_, UW_, SH_) run_tile<T, S, NC, VCAP, VCAP, UR_, UW_, UPD, CC, SH_>(lp, a, b, k)
+#define EMU_TILE(UR_, UW_, SH_)                                                        \
+  do {                                                                                \
+    if (lp.threads == kWideBlock)                                                     \
+      run_tile<T, S, NC, VCAP, VCAP, UR_, UW_, UPD, CC, SH_, kWideBlock>(lp, a, b, k); \
+    else                                                                              \
+      run_tile<T, S, NC, VCAP, VCAP, UR_, UW_, UPD, CC, SH_, kBlock>(lp, a, b, k);     \
+  } while (0)
     if (sh) {
       if (!ur && !uw) EMU_TILE(false, false, true);
       else if (!ur) EMU_TILE(false, kUA, true);

@@ -101,6 +101,21 @@ def test_unaligned_runs(device, dt):
     assert seen_unaligned or np.dtype(dt).itemsize == 16
 
 
+def test_wide_block_tiles(device):
+    """fp32 tiles over 32 KiB of LDS run 512-thread blocks: full and ragged
+    tiles, exact and unaligned runs, with and without beta."""
+    for k, (size, perm) in enumerate([([256, 64, 512], [2, 1, 0]),
+                                      ([251, 64, 509], [2, 1, 0]),
+                                      ([254, 8, 8, 508], [3, 1, 2, 0])]):
+        for be in (0.0, 4.2):
+            for off in (0, 1):
+                info = _vs_oracle(np.float32, size, perm, 2.3, be, offset=off, seed=70 + k)
+                assert info.threads == 512 and info.lds_bytes > 32768, (size, info.lds_bytes)
+                assert bool(info.unaligned_r or info.unaligned_w) == (off == 1 or k > 0)
+    info = _vs_oracle(np.float64, [256, 64, 512], [2, 1, 0], 2.3, 4.2, seed=80)
+    assert info.threads == 256
+
+
 def test_benchmark_workloads_full_size(device):
     """The benchmark workloads at their full sizes, against the oracle."""
     info = _vs_oracle(np.float32, *shapes.CONFIG1, 1.0, 0.0)

@@ -111,6 +111,28 @@ def test_emulated_tile_shapes(tune):
             _check(dt, size, perm, 2.3, 4.2 if k % 2 else 0.0, tune=tune)
 
 
+@pytest.mark.parametrize("dt", shapes.DTYPES)
+def test_emulated_wide_blocks(dt):
+    # tiles over 32 KiB of LDS run 512-thread blocks: full, ragged and
+    # unaligned tiles, with and without beta
+    tune = {4: [64, 128, 8192, 256], 8: [32, 64, 4096, 256],
+            16: [32, 64, 2048, 256]}[np.dtype(dt).itemsize]
+    wide = 0
+    for size, perm in [([64, 3, 128], [2, 1, 0]), ([70, 5, 130], [2, 1, 0]),
+                       ([66, 2, 3, 129], [3, 1, 0, 2]), ([128, 2, 64], [2, 1, 0])]:
+        for be in (0.0, 4.2):
+            d = _check(dt, size, perm, 2.3, be, tune=tune)
+            lds = int(d.split(" lds=")[1].split("B")[0])
+            assert (" x 512 " in d) == (lds > 32768), d
+            wide += " x 512 " in d
+            _check(dt, size, perm, 2.3, be, tune=tune, offset=1)
+    assert wide >= 4
+    rc, err, d = E.plan(np.float32, *shapes.CONFIG2)
+    assert rc == 0 and "grid: 18432 x 512 " in d, d
+    rc, err, d = E.plan(np.float64, *shapes.CONFIG2)
+    assert rc == 0 and "grid: 73728 x 256 " in d, d
+
+
 def test_emulated_benchmark_shapes_reduced():
     # the benchmark workloads' perms on reduced extents (same fusion classes)
     _check(np.float32, [16, 12, 8, 4, 6, 8], shapes.CONFIG2[1], 2.3, 4.2)
