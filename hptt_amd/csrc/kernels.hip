@@ -41,8 +41,11 @@ __global__ __launch_bounds__(NTH) void k_tile(TileDesc d, const T *__restrict__ 
   __syncthreads();
   tile_read<T, VR, NT, kUcapPlan, UR, NTH>(d, tc, tid, (const HPTTX_GLOBAL T *)A, lds, tb);
   __syncthreads();
-  tile_write<T, S, NC, VW, UPD, CC, SH, NT, kUcapPlan, UW, NTH>(d, tc, tid, (HPTTX_GLOBAL T *)B,
-                                                                lds, tb, co);
+  if constexpr (quad_write<T, UW, SH>())
+    tile_write_quad<T, S, NC, UPD, CC, NT, NTH>(d, tc, tid, (HPTTX_GLOBAL T *)B, lds, tb, co);
+  else
+    tile_write<T, S, NC, VW, UPD, CC, SH, NT, kUcapPlan, UW, NTH>(d, tc, tid,
+                                                                  (HPTTX_GLOBAL T *)B, lds, tb, co);
 }
 
 template <typename T, typename S, int NC, int V, bool UA, bool UPD, bool CC, bool NT>

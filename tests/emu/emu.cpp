@@ -80,9 +80,13 @@ void run_tile(const LaunchPlan &lp, const T *A, T *B, const Coef<S> &k) {
     for (uint32_t tid = 0; tid < NTH; ++tid) tile_tables<NTH>(d, tc, tid, tb);
     for (uint32_t tid = 0; tid < NTH; ++tid)
       tile_read<T, VR, false, kUcapPlan, UR, NTH>(d, tc, tid, A, lds, tb);
-    for (uint32_t tid = 0; tid < NTH; ++tid)
-      tile_write<T, S, NC, VW, UPD, CC, SH, false, kUcapPlan, UW, NTH>(d, tc, tid, B, lds, tb,
-                                                                      k);
+    for (uint32_t tid = 0; tid < NTH; ++tid) {
+      if constexpr (quad_write<T, UW, SH>())
+        tile_write_quad<T, S, NC, UPD, CC, false, NTH>(d, tc, tid, B, lds, tb, k);
+      else
+        tile_write<T, S, NC, VW, UPD, CC, SH, false, kUcapPlan, UW, NTH>(d, tc, tid, B, lds, tb,
+                                                                        k);
+    }
     if (!emu::g_fault.empty()) return;
   }
 }
