@@ -40,7 +40,10 @@ __device__ __forceinline__ void tile_one(const TileDesc &d, uint32_t lt, uint32_
   __syncthreads();
   tile_read<T, VR, NT, UCAP, UR>(d, tc, tid, A, lds, tb);
   __syncthreads();
-  tile_write<T, S, NC, VW, UPD, CC, SH, NT, UCAP, UW>(d, tc, tid, B, lds, tb, co);
+  if constexpr (quad_write<T, UW, SH>())
+    tile_write_quad<T, S, NC, UPD, CC, NT, kBlock>(d, tc, tid, B, lds, tb, co);
+  else
+    tile_write<T, S, NC, VW, UPD, CC, SH, NT, UCAP, UW>(d, tc, tid, B, lds, tb, co);
   if (more) __syncthreads();   // LDS is reused by the block's next tile
 }
 

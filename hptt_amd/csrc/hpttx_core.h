@@ -148,7 +148,7 @@ struct TileDesc {
   // over the banks; swz_mask = 0 is the plain padded layout.
   uint32_t swz_mask, swz_shift;
   FastDiv div_nR, div_nC, div_nvr, div_TC, div_nvw, div_TR, div_VS;
-  FastDiv div_ng4;       // groups of 4 R indices per tile, ceil(TR / 4)
+  FastDiv div_ngw;       // groups of 16 / esize R indices per tile
   DimList R, C, O;
 };
 
@@ -564,66 +564,67 @@ HPTTX_HD void tile_write(const TileDesc &d, const TileCoord &tc, uint32_t tid,
   }
 }
 
-// Write pass by 4 x 4 blocks, for 4-byte elements with exact write runs and
-// VS = 1.  Item (o, g, q): R indices r = 4g .. 4g+3 (B rows) and C indices
-// c = 4q .. 4q+3 (the q-th 16-byte vector of each B run).  The thread reads
-// the 16-byte LDS chunk g of the four LDS rows c (four ds_read_b128), so the
-// block arrives as x[c][r]; B's vector for row r is column r of it.  It
-// issues B's four loads first (UPDATE), then the LDS reads, then four
-// stores.  Against the general write pass this is a quarter of the LDS
-// instructions and of the per-element address arithmetic.  Exact writes make
-// every C run a whole number of vectors (clen % 4 == 0); R indices past rlen
-// (a partial last group) are skipped.
+// Write pass by W x W blocks (W = 16 / sizeof(T): 4 for 4-byte and 2 for
+// 8-byte elements), for exact write runs and VS = 1.  Item (o, g, q): R
+// indices r = W*g .. W*g+W-1 (B rows) and C indices c = W*q .. W*q+W-1 (the
+// q-th 16-byte vector of each B run).  The thread reads the 16-byte LDS chunk
+// g of the W LDS rows c (W ds_read_b128), so the block arrives as x[c][r];
+// B's vector for row r is column r of it.  It issues B's W loads first
+// (UPDATE), then the LDS reads, then W stores.  Against the general write
+// pass this is 1/W of the LDS instructions and of the per-element address
+// arithmetic.  Exact writes make every C run a whole number of vectors
+// (clen % W == 0); R indices past rlen (a partial last group) are skipped.
 template <typename T, typename S, int NC, bool UPD, bool CC, bool NT, uint32_t NTH>
 HPTTX_HD void tile_write_quad(const TileDesc &d, const TileCoord &tc, uint32_t tid,
                               HPTTX_GLOBAL T *HPTTX_RESTRICT B, const T *lds,
                               const TileTabs &tb, const Coef<S> &k) {
-  static_assert(sizeof(T) == 4, "4 x 4 blocks of 4-byte elements");
-  const uint32_t nq = d.nvw, ng = d.div_ng4.d;
+  constexpr uint32_t W = 16 / sizeof(T);
+  static_assert(W == 2 || W == 4, "blocks of 4- or 8-byte elements");
+  const uint32_t nq = d.nvw, ng = d.div_ngw.d;
   const uint32_t items = nq * ng * tc.olen;
   for (uint32_t it = tid; it < items; it += NTH) {
     const uint32_t rest = fdiv(it, d.div_nvw);
     const uint32_t q = it - rest * nq;
-    const uint32_t o = fdiv(rest, d.div_ng4);
+    const uint32_t o = fdiv(rest, d.div_ngw);
     const uint32_t g = rest - o * ng;
-    const uint32_t c0 = 4 * q, r0 = 4 * g;
+    const uint32_t c0 = W * q, r0 = W * g;
     if (c0 >= tc.clen || r0 >= tc.rlen) continue;
-    const uint32_t nr = umin32(4, tc.rlen - r0);
+    const uint32_t nr = umin32(W, tc.rlen - r0);
     const int64_t bbase = (int64_t)tc.c0 + c0 + tb.BO[o];
-    int64_t boff[4];
-    T y[4][4];
+    int64_t boff[W];
+    T y[W][W];
 #pragma unroll
-    for (uint32_t j = 0; j < 4; ++j) {
+    for (uint32_t j = 0; j < W; ++j) {
       boff[j] = j < nr ? bbase + tb.BR[r0 + j] : 0;
-      if (UPD && j < nr) gload<T, 4, NT>(B + boff[j], y[j]);
+      if (UPD && j < nr) gload<T, W, NT>(B + boff[j], y[j]);
     }
-    T x[4][4];
+    T x[W][W];
 #pragma unroll
-    for (uint32_t m = 0; m < 4; ++m) {
+    for (uint32_t m = 0; m < W; ++m) {
       const uint32_t row = o * d.TC + c0 + m;
       const uint32_t chunk = g ^ ((row >> d.swz_shift) & d.swz_mask);
-      vload<T, 4>(lds + row * d.lds_row + chunk * 4, x[m]);
+      vload<T, W>(lds + row * d.lds_row + chunk * W, x[m]);
     }
 #pragma unroll
-    for (uint32_t j = 0; j < 4; ++j) {
+    for (uint32_t j = 0; j < W; ++j) {
       if (j >= nr) continue;
-      T out[4];
+      T out[W];
 #pragma unroll
-      for (uint32_t m = 0; m < 4; ++m) {
+      for (uint32_t m = 0; m < W; ++m) {
         const Elem<S, NC> &xe = reinterpret_cast<const Elem<S, NC> &>(x[m][j]);
         const Elem<S, NC> &ye = reinterpret_cast<const Elem<S, NC> &>(y[j][m]);
         Elem<S, NC> re = axpby<S, NC, UPD, CC>(xe, ye, k);
         __builtin_memcpy(&out[m], &re, sizeof(T));
       }
-      gstore<T, 4, NT>(B + boff[j], out);
+      gstore<T, W, NT>(B + boff[j], out);
     }
   }
 }
 
-// The 4 x 4 write pass applies: 4-byte elements, exact writes, VS == 1.
+// The W x W write pass applies: 4- or 8-byte elements, exact writes, VS == 1.
 template <typename T, bool UW, bool SH>
 HPTTX_HD constexpr bool quad_write() {
-  return sizeof(T) == 4 && !UW && !SH;
+  return (sizeof(T) == 4 || sizeof(T) == 8) && !UW && !SH;
 }
 
 // ---------------------------------------------------------------------------

@@ -419,7 +419,7 @@ void finalize_vectors(const Problem &p, LaunchPlan &lp, uint32_t vr, uint32_t vw
     t.nvw = (t.TC * t.VS + cap - 1) / cap;
     t.div_nvr = make_fastdiv(t.nvr);
     t.div_nvw = make_fastdiv(t.nvw);
-    t.div_ng4 = make_fastdiv((t.TR + 3) / 4);
+    t.div_ngw = make_fastdiv((t.TR + cap - 1) / cap);
     // LDS rows: a multiple of the read vector.  Rows of a power-of-two
     // number (>= 4) of vectors use the chunk swizzle (measured 1.4% faster
     // than padding on the 6-D fp32 workload); others are padded by one
