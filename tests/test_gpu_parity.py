@@ -101,6 +101,23 @@ def test_unaligned_runs(device, dt):
     assert seen_unaligned or np.dtype(dt).itemsize == 16
 
 
+@pytest.mark.parametrize("dt", shapes.DTYPES)
+def test_random_large_shapes(device, dt):
+    """Random ranks 2-4 with every extent in [24, 400) and up to 6M elements:
+    many full-size tiles next to ragged edges, on aligned and
+    one-element-offset pointers, with and without beta."""
+    rng = np.random.default_rng(500 + shapes.DTYPES.index(dt))
+    for k in range(10):
+        rank = int(rng.integers(2, 5))
+        while True:
+            size = [int(v) for v in rng.integers(24, 400, size=rank)]
+            if int(np.prod(size)) <= 6 * 2**20:
+                break
+        perm = [int(v) for v in rng.permutation(rank)]
+        al, be = [(1.0, 0.0), (2.3, 4.2)][k % 2]
+        _vs_oracle(dt, size, perm, al, be, offset=(k // 2) % 2, seed=300 + k)
+
+
 def test_wide_block_tiles(device):
     """fp32 tiles over 32 KiB of LDS run 512-thread blocks: full and ragged
     tiles, exact and unaligned runs, with and without beta."""
