@@ -40,12 +40,12 @@ WORKLOADS = {
     "3d-f32-copy": dict(size=[64, 128, 256], perm=[2, 1, 0], dtype="float32",
                         alpha=1.0, beta=0.0, flush=True),
     # identity perm (axpby path): a plain streaming copy of 2 x 512 MiB, the
-    # practical ceiling the transposes are compared with
+    # streaming reference the transposes are compared with
     "copy-f32-512m": dict(size=[131072, 1024], perm=[0, 1], dtype="float32",
                           alpha=1.0, beta=0.0, flush=False),
     # identity perm with beta on the main workload's element count: the same
     # bytes and the same 2:1 read:write mix as the 6-D transposes, with no
-    # transposition -- their practical ceiling
+    # transposition -- their streaming reference
     "axpby-f32-151m": dict(size=[12288, 12288], perm=[0, 1], dtype="float32",
                            alpha=2.3, beta=4.2, flush=False),
 }

@@ -118,19 +118,23 @@ def test_random_large_shapes(device, dt):
         _vs_oracle(dt, size, perm, al, be, offset=(k // 2) % 2, seed=300 + k)
 
 
-def test_wide_block_tiles(device):
-    """fp32 tiles over 32 KiB of LDS run 512-thread blocks: full and ragged
-    tiles, exact and unaligned runs, with and without beta."""
+@pytest.mark.parametrize("dt", shapes.DTYPES)
+def test_wide_block_tiles(device, dt):
+    """Tiles over 32 KiB of LDS (the fp32 64x128 and 8-byte 32x128 tiles) run
+    512-thread blocks: full and ragged tiles, exact and unaligned runs, with
+    and without beta; complex<double> tiles stay at 256 threads."""
+    wide = np.dtype(dt).itemsize <= 8
     for k, (size, perm) in enumerate([([256, 64, 512], [2, 1, 0]),
                                       ([251, 64, 509], [2, 1, 0]),
                                       ([254, 8, 8, 508], [3, 1, 2, 0])]):
         for be in (0.0, 4.2):
             for off in (0, 1):
-                info = _vs_oracle(np.float32, size, perm, 2.3, be, offset=off, seed=70 + k)
-                assert info.threads == 512 and info.lds_bytes > 32768, (size, info.lds_bytes)
-                assert bool(info.unaligned_r or info.unaligned_w) == (off == 1 or k > 0)
-    info = _vs_oracle(np.float64, [256, 64, 512], [2, 1, 0], 2.3, 4.2, seed=80)
-    assert info.threads == 256
+                info = _vs_oracle(dt, size, perm, 2.3, be, offset=off, seed=70 + k)
+                assert (info.threads == 512) == wide, (size, info.threads)
+                assert (info.lds_bytes > 32768) == wide, (size, info.lds_bytes)
+                if wide and off == 0:
+                    assert bool(info.unaligned_r or info.unaligned_w) == (k == 1 or (
+                        k == 2 and np.dtype(dt).itemsize == 4))
 
 
 def test_benchmark_workloads_full_size(device):

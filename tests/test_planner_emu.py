@@ -130,7 +130,9 @@ def test_emulated_wide_blocks(dt):
     rc, err, d = E.plan(np.float32, *shapes.CONFIG2)
     assert rc == 0 and "grid: 18432 x 512 " in d, d
     rc, err, d = E.plan(np.float64, *shapes.CONFIG2)
-    assert rc == 0 and "grid: 73728 x 256 " in d, d
+    assert rc == 0 and "C=1024/128 " in d and "grid: 36864 x 512 " in d, d
+    rc, err, d = E.plan(np.complex128, *shapes.CONFIG3_TILE)
+    assert rc == 0 and "grid: 165888 x 256 " in d, d
 
 
 def test_emulated_benchmark_shapes_reduced():
