@@ -85,7 +85,7 @@ int create_common(int dtype, const void *in, void *out, uint32_t rank,
   p->update = p->cc ? (std::fabs(br) >= 1e-16 || std::fabs(bi) >= 1e-16) : update_real(br);
   TuneParams tp{};
   if (!tune_lookup(p->prob, p->update, tp) && tuning_timeout != 0.0) {
-    rc = autotune(p->prob, p->coef, p->update, p->cc, tuning_timeout, p->stream, tp, err);
+    rc = autotune(p->prob, p->coef, p->update, p->cc, tuning_timeout, p->stream, tp, err, in, out);
     if (rc != kOk) {
       delete p;
       return fail(rc, err);

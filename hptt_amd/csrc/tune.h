@@ -27,9 +27,11 @@ std::string tune_key(const Problem &p, bool update);
 bool tune_lookup(const Problem &p, bool update, TuneParams &out);
 
 // Time candidate tile shapes and keep the fastest; records it in the cache.
+// The candidates run on scratch copies of the caller's A and B (src_a /
+// src_b, read only; zero-filled scratch when null), never on the caller's B.
 int autotune(const Problem &p, const CoefD &coef, bool update, bool cc,
              double timeout_s, hipStream_t stream, TuneParams &out,
-             std::string &err);
+             std::string &err, const void *src_a = nullptr, const void *src_b = nullptr);
 
 // The candidate list the tuner walks (model default first).
 std::vector<TuneParams> tune_candidates(const Problem &p);
