@@ -438,7 +438,7 @@ def main():
         return
     if args.workload.startswith("batched-"):
         res = time_batched({"batched-f32": "float32", "batched-f64": "float64"}[args.workload],
-                           reps=max(1, args.steps))
+                           reps=max(1, args.steps), tune=args.tune)
         print(json.dumps({"workload": args.workload, **res}), flush=True)
         return
     log("bench: main workload %s, %d steps" % (args.workload, args.steps))

@@ -218,9 +218,15 @@ int tune_class_batch(const PlannedItem &rep, const BatchItemSpec &spec, double b
   uint32_t *d_begins = nullptr, *d_ent_of = nullptr;
   size_t ent_of_cap = 0;
   int rc = kOk;
-  if (hipMemsetAsync(A, 0, K * a_bytes, st) != hipSuccess ||
-      hipMemsetAsync(B, 0, K * b_bytes, st) != hipSuccess ||
-      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+  // Every scratch copy starts as the member's own A and B (read only): timings
+  // on zero-filled buffers favour other shapes than real data does.
+  bool filled = true;
+  for (uint64_t k = 0; k < K && filled; ++k)
+    filled = hipMemcpyAsync(A + k * a_bytes, spec.A, rep.p.spanA * es, hipMemcpyDeviceToDevice,
+                            st) == hipSuccess &&
+             hipMemcpyAsync(B + k * b_bytes, spec.B, rep.p.spanB * es, hipMemcpyDeviceToDevice,
+                            st) == hipSuccess;
+  if (!filled || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
       hipMalloc(&d_ent, sizeof(BatchEntry) * K) != hipSuccess ||
       hipMalloc(&d_begins, sizeof(uint32_t) * K) != hipSuccess) {
     err = "batch tuning: HIP setup failed";
