@@ -71,6 +71,36 @@ def test_python_binding_loads_and_validates(lib):
     assert hptt_amd.create_plan(fake, fake, [4], [0], 1.0, 0.0, dtype="s") is None
 
 
+def test_python_structs_match_the_c_abi(tmp_path):
+    """Every ctypes Structure of the Python binding has the size and field
+    offsets of its struct in include/hptt_amd.h (measured by the host C
+    compiler), so the binding cannot read or write past a field."""
+    import hptt_amd
+    pairs = [(hptt_amd.PlanInfo, "hpttx_plan_info"), (hptt_amd.BatchItem, "hpttx_batch_item"),
+             (hptt_amd.BatchInfo, "hpttx_batch_info"),
+             (hptt_amd.ServerConfigC, "hpttx_server_config"),
+             (hptt_amd.Completion, "hpttx_completion"), (hptt_amd.ServerStats, "hpttx_server_stats")]
+    cname = lambda f: "in" if f == "in_" else f
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hptt_amd.h"', 'int main(void) {']
+    for cls, cs in pairs:
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cs, cs))
+        for f, _ in cls._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cs, f, cs, cname(f)))
+    lines += ['return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    r = subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                        str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cls, cs in pairs:
+        assert got[(cs, "size")] == ctypes.sizeof(cls), cs
+        for f, _ in cls._fields_:
+            assert got[(cs, f)] == getattr(cls, f).offset, (cs, f)
+
+
 def test_cpp_dropin_header_compiles(tmp_path):
     """include/hptt/hptt.h compiles as plain C++17 against the C ABI header
     (host compiler, no HIP headers needed)."""
