@@ -99,6 +99,21 @@ def test_python_structs_match_the_c_abi(tmp_path):
         assert got[(cs, "size")] == ctypes.sizeof(cls), cs
         for f, _ in cls._fields_:
             assert got[(cs, f)] == getattr(cls, f).offset, (cs, f)
+    # and the same fields, in order (a missing trailing field could hide in padding)
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "hptt_amd.h")).read(),
+                  flags=re.S)
+    for cls, cs in pairs:
+        body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (cs, cs), text, flags=re.S).group(1)
+        names = []
+        for decl in body.split(";"):
+            decl = decl.strip()
+            if not decl:
+                continue
+            head, _, rest = decl.partition(" ")
+            while rest.startswith(("int", "uint", "double", "void", "*", "const")) and " " in rest:
+                head, _, rest = rest.partition(" ")
+            names += [re.sub(r"[\s*]|\[.*\]", "", n) for n in rest.split(",")]
+        assert [cname(f) for f, _ in cls._fields_] == names, (cs, names)
 
 
 def test_cpp_dropin_header_compiles(tmp_path):
