@@ -241,6 +241,86 @@ def time_batched(dtype_name, reps=5, seq_reps=3, tune=0.0):
     return res
 
 
+def time_sweep(reps=10):
+    """The reference benchmark's 102 workloads (fp32, alpha=2.3, beta=4.2,
+    test/benchmark/inc/hptt/benchmark/benchmark_trans.h:24-140): effective
+    GB/s of plain plan->exec() per shape (median of reps event-timed execs),
+    with the minimum and median over shapes and the slowest shapes."""
+    import torch
+
+    import hptt_amd
+    from hptt_amd.workloads import (REFERENCE_SWEEP_ALPHA, REFERENCE_SWEEP_BETA,
+                                    reference_sweep)
+    cases = reference_sweep()
+    nmax = max(int(np.prod(s)) for _, s, _ in cases)
+    g = torch.Generator(device="cuda").manual_seed(102)
+    Abuf = torch.empty(nmax, device="cuda").uniform_(-1, 1, generator=g)
+    Bbuf = torch.empty(nmax, device="cuda").uniform_(-1, 1, generator=g)
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    rows = {}
+    for name, size, perm in cases:
+        n = int(np.prod(size))
+        plan = hptt_amd.create_plan_checked(Abuf[:n], Bbuf[:n], size, perm, REFERENCE_SWEEP_ALPHA,
+                                            REFERENCE_SWEEP_BETA, stream=stream)
+        info = plan.info()
+        plan.exec()
+        evs = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            plan.exec()
+            b.record(stream)
+            evs.append((a, b))
+        torch.cuda.synchronize()
+        ms = float(np.median([a.elapsed_time(b) for a, b in evs]))
+        gbps = info.algorithmic_bytes / (ms * 1e-3) / 1e9
+        rows[name] = {"size": size, "perm": perm, "gbps": round(gbps, 1),
+                      "pct_hbm_peak": round(100 * gbps / PEAK_GBPS, 1),
+                      "path": hptt_amd.PATHS[info.path]}
+        del plan
+    rates = sorted(r["gbps"] for r in rows.values())
+    worst = sorted(rows.items(), key=lambda kv: kv[1]["gbps"])[:5]
+    del Abuf, Bbuf
+    torch.cuda.empty_cache()
+    return {"shapes": len(rows), "min_gbps": rates[0], "median_gbps": float(np.median(rates)),
+            "max_gbps": rates[-1], "min_pct": round(100 * rates[0] / PEAK_GBPS, 1),
+            "median_pct": round(100 * float(np.median(rates)) / PEAK_GBPS, 1),
+            "slowest": [dict(name=k, **v) for k, v in worst], "per_shape": rows}
+
+
+def cpu_batched_baseline(dtype_name, budget_s=5.0):
+    """The oracle (port of the reference's RefTrans, OpenMP) over the batched
+    manifest in order, one tensor after another, for about budget_s seconds:
+    the CPU's sequential transposes/s on the same tensors."""
+    import oracle
+    from hptt_amd.workloads import batch_manifest
+    man = batch_manifest()
+    dt = np.dtype(dtype_name)
+    threads, aff, physical = host_cores()
+    done, nbytes, t_cpu = 0, 0, 0.0
+    t_start = time.perf_counter()
+    for k, m in enumerate(man):
+        if time.perf_counter() - t_start > budget_s:
+            break
+        n = int(np.prod(m["in_size"]))
+        A = oracle.uniform(n, dt, 2 * k + 1, nthreads=threads)
+        B = oracle.uniform(n, dt, 2 * k + 2, nthreads=threads)
+        t0 = time.perf_counter()
+        assert oracle.transpose(A, B, m["in_size"], m["perm"], m["alpha"], m["beta"],
+                                nthreads=threads) == 0
+        t_cpu += time.perf_counter() - t0
+        done += 1
+        nbytes += n * dt.itemsize * (3 if abs(m["beta"]) >= 1e-16 else 2)
+    return {"value": round(done / t_cpu, 1), "unit": "transposes/s", "cores": threads,
+            "affinity_cpus": aff, "physical_cores_host": physical, "kind": "port",
+            "gbps": round(nbytes / t_cpu / 1e9, 2),
+            "sample": "the first %d of 4096 manifest tensors (%s) in order, timed per transpose "
+                      "(data generation excluded); oracle OpenMP loop, %d threads"
+                      % (done, dtype_name, threads)}
+
+
 def serving_trace(count, seed=5150):
     """Serving requests: total elements log-uniform in [1K, 64M], rank 2..6
     with dims splitting that count, random perm, dtype uniform over the four,
@@ -305,18 +385,32 @@ def time_serving(count=600, rates=(0.5, 1.0, 2.0), inflight_gb=8.0):
     stream = torch.cuda.Stream()
     sp = int(stream.cuda_stream)
 
-    def seq_run(arrivals):
-        """One request at a time: create the plan, exec, wait."""
+    # Plain exec() baseline: every request's plan created beforehand (plans
+    # kept resident, as the server keeps its own), then exec + wait per
+    # request in arrival order.
+    plans = [hptt_amd.create_plan_checked(a, b, r["in_size"], r["perm"], r["alpha"], r["beta"],
+                                          stream=sp) for r, (a, b) in zip(trace, args)]
+    exec_fn, sync_fn = L.hpttx_plan_exec, L.hpttx_plan_sync
+    handles = [p._h for p in plans]
+
+    def seq_run(arrivals, create=False):
+        """One request at a time: exec, wait (create=True: also create the
+        plan per request, inside the timed region)."""
         t0 = time.perf_counter()
         lat = []
         for k, (r, (a, b)) in enumerate(zip(trace, args)):
             due = t0 + arrivals[k]
             while time.perf_counter() < due:
                 pass
-            p = hptt_amd.create_plan_checked(a, b, r["in_size"], r["perm"], r["alpha"], r["beta"],
-                                             stream=sp)
-            p.exec()
-            p.sync()
+            if create:
+                p = hptt_amd.create_plan_checked(a, b, r["in_size"], r["perm"], r["alpha"],
+                                                 r["beta"], stream=sp)
+                p.exec()
+                p.sync()
+            else:
+                h = handles[k]
+                if exec_fn(h) != 0 or sync_fn(h) != 0:
+                    raise RuntimeError(hptt_amd.last_error())
             lat.append(time.perf_counter() - due)
         return time.perf_counter() - t0, lat
 
@@ -352,18 +446,32 @@ def time_serving(count=600, rates=(0.5, 1.0, 2.0), inflight_gb=8.0):
                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}
 
     zero = [0.0] * len(trace)
-    seq_run(zero[:0] + zero)          # warm-up (plans, kernels)
+    seq_run(zero)                     # warm-up (kernels)
     seq_wall, seq_lat = seq_run(zero)
+    seqc_wall, seqc_lat = seq_run(zero, create=True)
     cap = len(trace) / seq_wall
     serve_run(zero)                   # warm-up
     sw, sl, launches, _ = serve_run(zero)
-    out = {"requests": len(trace),
-           "bytes": int(sum(r["n"] * [4, 8, 8, 16][r["code"]] * (2 + (r["beta"] != 0)) for r in trace)),
-           "sequential_closed": summary(seq_wall, seq_lat),
-           "served_closed": dict(summary(sw, sl), launches=launches),
-           "inflight_budget_gb": inflight_gb}
+    nbytes = int(sum(r["n"] * [4, 8, 8, 16][r["code"]] * (2 + (abs(r["beta"]) >= 1e-16))
+                     for r in trace))
+    bound_s = nbytes / (PEAK_GBPS * 1e9)
+    out = {"requests": len(trace), "bytes": nbytes,
+           "byte_bound_ms": round(bound_s * 1e3, 3),
+           "sequential_closed": dict(summary(seq_wall, seq_lat),
+                                     bound_frac=round(bound_s / seq_wall, 4)),
+           "sequential_create_exec_closed": summary(seqc_wall, seqc_lat),
+           "served_closed": dict(summary(sw, sl), launches=launches,
+                                 gbps=round(nbytes / sw / 1e9, 1),
+                                 bound_frac=round(bound_s / sw, 4)),
+           "inflight_budget_gb": inflight_gb,
+           "note": "sequential = plans created beforehand, then exec + wait per request; "
+                   "sequential_create_exec also creates each plan inside the timed loop; "
+                   "bound_frac = (bytes / 8 TB/s) / wall time"}
     out["served_vs_sequential_throughput"] = round(
         out["served_closed"]["transposes_per_s"] / out["sequential_closed"]["transposes_per_s"], 2)
+    out["served_vs_create_exec_throughput"] = round(
+        out["served_closed"]["transposes_per_s"] /
+        out["sequential_create_exec_closed"]["transposes_per_s"], 2)
     rng = np.random.default_rng(99)
     for f in rates:
         lam = f * cap
@@ -375,9 +483,30 @@ def time_serving(count=600, rates=(0.5, 1.0, 2.0), inflight_gb=8.0):
                                  "sequential": summary(w1, l1),
                                  "served": dict(summary(w2, l2), launches=n2,
                                                 max_submit_lag_ms=round(lag * 1e3, 3))}
-    del pools_a, pools_b, items, args
+    del pools_a, pools_b, items, args, plans, handles
     torch.cuda.empty_cache()
     return out
+
+
+def host_cores():
+    """(threads used, CPUs this process may run on, physical cores of the
+    host per lscpu).  Threads: every CPU in the affinity mask, capped by
+    OMP_NUM_THREADS when that is set (the share of the host a job gets)."""
+    aff = len(os.sched_getaffinity(0))
+    threads = aff
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        threads = min(threads, int(env))
+    physical = None
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {l.split(":", 1)[0].strip(): l.split(":", 1)[1].strip() for l in out.splitlines()
+              if ":" in l}
+        physical = int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except Exception:
+        pass
+    return threads, aff, physical
 
 
 def cpu_baseline(budget_s=10.0, name="6d-f32"):
@@ -387,7 +516,7 @@ def cpu_baseline(budget_s=10.0, name="6d-f32"):
     w = WORKLOADS[name]
     n = int(np.prod(w["size"]))
     dt = np.dtype(w["dtype"])
-    threads = min(os.cpu_count() or 1, 16)
+    threads, aff, physical = host_cores()
     A = oracle.uniform(n, dt, 1, nthreads=threads)
     B = oracle.uniform(n, dt, 2, nthreads=threads)
     cc = isinstance(w["alpha"], complex) or isinstance(w["beta"], complex)
@@ -403,6 +532,7 @@ def cpu_baseline(budget_s=10.0, name="6d-f32"):
     med = float(np.median(times))
     nbytes = n * dt.itemsize * (3 if upd else 2)
     return {"value": round(nbytes / med / 1e9, 2), "unit": "GB/s", "cores": threads,
+            "affinity_cpus": aff, "physical_cores_host": physical,
             "kind": "port", "best_gbps": round(nbytes / min(times) / 1e9, 2),
             "sample": "%s (%s) repeated for %.1f s: %d runs, median; oracle OpenMP loop "
                       "(reference RefTrans order), %d threads"
@@ -432,6 +562,9 @@ def main():
         sys.exit(1)
     import hptt_amd
     hptt_amd.lib()
+    if args.workload == "sweep":
+        print(json.dumps({"workload": "reference-sweep", **time_sweep()}), flush=True)
+        return
     if args.workload == "serving":
         res = time_serving()
         print(json.dumps({"workload": "serving", **res}), flush=True)
@@ -455,6 +588,10 @@ def main():
                 continue
             secondary[name] = time_workload(name, max(10, args.steps // 2), args.warmup, args.tune)
             log("bench: %s %.1f GB/s" % (name, secondary[name]["gbps"]))
+        secondary["reference-sweep"] = time_sweep()
+        sw = secondary["reference-sweep"]
+        log("bench: reference sweep (%d shapes) min %.1f median %.1f GB/s; slowest %s"
+            % (sw["shapes"], sw["min_gbps"], sw["median_gbps"], sw["slowest"][0]["name"]))
         for name in ("batched-f32", "batched-f64"):
             secondary[name] = time_batched(name.split("-")[1].replace("f", "float"))
             log("bench: %s %s transposes/s batched vs %s sequential (x%.2f), identical=%s"
@@ -480,6 +617,10 @@ def main():
                 c2 = cpu_baseline(2.0, name)
                 secondary.setdefault("cpu_baseline", {})[name] = c2
                 log("bench: CPU %s %.2f GB/s" % (name, c2["value"]))
+            for name in ("batched-f32", "batched-f64"):
+                c3 = cpu_batched_baseline(name.split("-")[1].replace("f", "float"), 3.0)
+                secondary.setdefault("cpu_baseline", {})[name] = c3
+                log("bench: CPU %s %.1f transposes/s" % (name, c3["value"]))
     gbps = main_res["gbps"]
     line = {
         "metric": METRIC,

@@ -214,6 +214,53 @@ def _stream_ptr(stream):
     return int(stream.cuda_stream)  # torch.cuda.Stream
 
 
+def _check_shape_args(in_size, perm, in_outer_size, out_outer_size):
+    """Array lengths the C ABI relies on (it reads rank entries of each):
+    in_size as long as perm, each outer size empty or as long as perm
+    (the reference returns nullptr on a mismatch, src/hptt.cc:35-39)."""
+    rank = len(perm)
+    if len(in_size) != rank:
+        raise HpttError(EINVAL, "in_size has %d entries but perm has %d" % (len(in_size), rank))
+    for name, o in (("in_outer_size", in_outer_size), ("out_outer_size", out_outer_size)):
+        if o is not None and len(o) not in (0, rank):
+            raise HpttError(EINVAL, "%s has %d entries; expected 0 or %d" % (name, len(o), rank))
+
+
+def _spans(in_size, perm, in_outer_size, out_outer_size):
+    """Elements A and B must hold (1 + sum (extent-1)*stride, column-major
+    over the outer sizes), or None when the shape is invalid (the C ABI then
+    says why)."""
+    rank = len(perm)
+    try:
+        size = [int(v) for v in in_size]
+        pm = [int(v) for v in perm]
+    except (TypeError, ValueError):
+        return None
+    if sorted(pm) != list(range(rank)) or any(v < 1 for v in size):
+        return None
+    out_size = [size[p] for p in pm]
+    io = [int(v) for v in in_outer_size] if in_outer_size else size
+    oo = [int(v) for v in out_outer_size] if out_outer_size else out_size
+    span = []
+    for ext, outer in ((size, io), (out_size, oo)):
+        s, stride = 1, 1
+        for d in range(rank):
+            s += (ext[d] - 1) * stride
+            stride *= outer[d]
+        span.append(s)
+    return span
+
+
+def _check_tensor_spans(in_data, out_data, in_size, perm, in_outer_size, out_outer_size):
+    sp = _spans(in_size, perm, in_outer_size, out_outer_size)
+    if sp is None:
+        return
+    for name, t, need in (("A", in_data, sp[0]), ("B", out_data, sp[1])):
+        if t is not None and not isinstance(t, int) and t.numel() < need:
+            raise HpttError(EINVAL, "%s holds %d elements; the shape needs %d" % (
+                name, t.numel(), need))
+
+
 def create_plan_checked(in_data, out_data, in_size, perm, alpha, beta,
                         num_threads: int = 0, tuning_timeout: float = 0.0,
                         in_outer_size=(), out_outer_size=(), stream=None,
@@ -225,6 +272,8 @@ def create_plan_checked(in_data, out_data, in_size, perm, alpha, beta,
     if code_a != code_b:
         raise TypeError("A and B must have the same dtype")
     code = code_a
+    _check_shape_args(in_size, perm, in_outer_size, out_outer_size)
+    _check_tensor_spans(in_data, out_data, in_size, perm, in_outer_size, out_outer_size)
     rank = len(perm)
     out = ctypes.c_void_p()
     L = lib()
@@ -271,6 +320,13 @@ class BatchItem(ctypes.Structure):
         ("complex_coef", ctypes.c_int32), ("reserved_", ctypes.c_int32)]
 
 
+class BatchItemInfo(ctypes.Structure):
+    """include/hptt_amd.h hpttx_batch_item_info."""
+    _fields_ = [("group", ctypes.c_uint32), ("kernel", ctypes.c_uint32),
+                ("variant", ctypes.c_uint32), ("lds_bytes", ctypes.c_uint32),
+                ("tiles", ctypes.c_uint32), ("reserved_", ctypes.c_uint32)]
+
+
 class BatchInfo(ctypes.Structure):
     _fields_ = [("count", ctypes.c_uint32), ("groups", ctypes.c_uint32),
                 ("algorithmic_bytes", ctypes.c_uint64), ("tiles", ctypes.c_uint64),
@@ -293,6 +349,8 @@ def _bind_batch(L):
     L.hpttx_batch_use_graph.argtypes = [vp, ctypes.c_int]
     L.hpttx_batch_get_info.restype = ctypes.c_int
     L.hpttx_batch_get_info.argtypes = [vp, ctypes.POINTER(BatchInfo)]
+    L.hpttx_batch_get_item_info.restype = ctypes.c_int
+    L.hpttx_batch_get_item_info.argtypes = [vp, u32, ctypes.POINTER(BatchItemInfo)]
     L.hpttx_batch_item_path.restype = ctypes.c_int
     L.hpttx_batch_item_path.argtypes = [vp, u32, ctypes.c_char_p, ctypes.c_size_t]
     L.hpttx_batch_destroy.restype = None
@@ -311,6 +369,10 @@ def make_batch_item(in_data, out_data, in_size, perm, alpha, beta, dtype=None,
         b, cb = _addr_and_dtype(out_data, code)
     if ca != cb:
         raise TypeError("A and B must have the same dtype")
+    _check_shape_args(in_size, perm, in_outer_size, out_outer_size)
+    if len(perm) > 6:
+        raise HpttError(EUNSUPPORTED, "rank above 6 is not supported")
+    _check_tensor_spans(in_data, out_data, in_size, perm, in_outer_size, out_outer_size)
     it = BatchItem()
     it.dtype = ca
     it.rank = len(perm)
@@ -370,6 +432,11 @@ class Batch:
     def info(self) -> BatchInfo:
         inf = BatchInfo()
         _check(lib().hpttx_batch_get_info(self._h, ctypes.byref(inf)))
+        return inf
+
+    def item_info(self, index: int) -> BatchItemInfo:
+        inf = BatchItemInfo()
+        _check(lib().hpttx_batch_get_item_info(self._h, index, ctypes.byref(inf)))
         return inf
 
     def item_path(self, index: int) -> str:

@@ -132,9 +132,14 @@ struct Batch {
   std::vector<int> lane;            // stream of each group: 0 exec, i+1 side[i]
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // Recorded after every exec, on whichever stream it ran: set_data and
+  // destroy wait on it before they touch the device tables.
+  hipEvent_t last_exec = nullptr;
   bool use_graph = true;
   uint64_t bytes = 0;
   std::vector<std::string> paths;   // per user item
+  std::vector<uint32_t> item_group; // per user item: its group
+  std::vector<uint32_t> item_pos;   // per user item: its entry within the group
 };
 
 // Plan every item (tile shapes tuned per class when tuning_timeout != 0).

@@ -16,6 +16,7 @@
 
 #include "batch.h"
 #include "batch_kernels.h"
+#include "device.h"
 #include "hpttx_core.h"
 #include "tune.h"
 
@@ -355,6 +356,8 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
   b.count = (uint32_t)items.size();
   b.stream = stream;
   b.paths.resize(items.size());
+  b.item_group.assign(items.size(), 0);
+  b.item_pos.assign(items.size(), 0);
   std::vector<PlannedItem> pl(items.size());
   std::vector<std::string> keys(items.size());
   std::map<std::string, size_t> class_rep;   // largest member of each class
@@ -365,6 +368,11 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
     auto it2 = class_rep.find(keys[k]);
     if (it2 == class_rep.end() || pl[it2->second].p.N < pl[k].p.N) class_rep[keys[k]] = k;
   }
+  // Items are validated first, then the device (as for single plans).
+  {
+    int rc = require_target_device(err);
+    if (rc != kOk) return rc;
+  }
   // Per-class tile shapes: tuned in batch context on copies of the class's
   // largest member (scratch buffers), or taken from the tuning cache (under
   // a batch key: a lone plan of the same problem may tune differently).
@@ -373,7 +381,7 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
     std::map<std::string, TuneParams> class_tune;
     for (auto &kv : class_rep) {
       const PlannedItem &rep = pl[kv.second];
-      const std::string key = "batch|" + tune_key(rep.p, rep.update);
+      const std::string key = "batch|" + tune_key(rep.p, rep.update, rep.cc);
       TuneParams tp{};
       if (!tune_lookup_key(key, tp)) {
         int rc = tune_class_batch(rep, items[kv.second], share, stream, tp, err);
@@ -450,6 +458,10 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
       he = hipMemcpy(g.d_ent_of, ent_of.data(), sizeof(uint32_t) * ent_of.size(),
                      hipMemcpyHostToDevice);
     b.bytes += g.bytes;
+    for (uint32_t i = 0; i < (uint32_t)g.order.size(); ++i) {
+      b.item_group[g.order[i]] = (uint32_t)b.groups.size();
+      b.item_pos[g.order[i]] = i;
+    }
     b.groups.push_back(std::move(g));
     if (he != hipSuccess) break;
   }
@@ -466,6 +478,7 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
     he = hipEventCreateWithFlags(&e, hipEventDisableTiming);
     if (he == hipSuccess) b.events.push_back(e);
   }
+  if (he == hipSuccess) he = hipEventCreateWithFlags(&b.last_exec, hipEventDisableTiming);
   if (he == hipSuccess) {
     std::vector<size_t> gorder(b.groups.size());
     for (size_t i = 0; i < gorder.size(); ++i) gorder[i] = i;
@@ -491,8 +504,6 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
 
 hipError_t batch_enqueue(Batch &b, hipStream_t st) {
   if (b.groups.empty()) return hipSuccess;
-  // Counters are cleared by a kernel (not a memset node) so the sequence can
-  // be captured in a graph and replayed.
   hipError_t e = hipSuccess;
   if (!b.side.empty()) e = hipEventRecord(b.events[0], st);
   for (size_t i = 0; i < b.side.size() && e == hipSuccess; ++i)
@@ -545,6 +556,8 @@ void batch_free(Batch &b) {
   }
   for (hipStream_t s : b.side) (void)hipStreamDestroy(s);
   for (hipEvent_t e : b.events) (void)hipEventDestroy(e);
+  if (b.last_exec) (void)hipEventDestroy(b.last_exec);
+  b.last_exec = nullptr;
   b.side.clear();
   b.events.clear();
   b.gexec = nullptr;
@@ -554,32 +567,32 @@ void batch_free(Batch &b) {
 
 int batch_set_data(Batch &b, uint32_t k, const void *A, void *B, std::string &err) {
   if (!A || !B) { err = "null data pointer"; return kEInval; }
-  for (BatchGroup &g : b.groups) {
-    for (uint32_t i = 0; i < g.nent; ++i) {
-      if (g.order[i] != k) continue;
-      BatchEntry &e = g.h_ent[i];
-      const uint32_t es = dtype_esize(g.dtype);
-      // Exact-vector sides were chosen for the old pointers' 16-byte
-      // alignment; new pointers must keep it there (unaligned sides accept
-      // any element-aligned pointer).
-      const bool vr = (e.variant & kVarVr) != 0, vw = (e.variant & kVarVw) != 0;
-      const bool rows = (e.variant & kVarRows) != 0;
-      const bool need_a = vr, need_b = rows ? vr : vw;
-      const uint32_t cap = 16 / es;
-      if ((need_a && align_vec(A, es, cap) != cap) || (need_b && align_vec(B, es, cap) != cap)) {
-        err = "new pointers are not 16-byte aligned like the ones the batch was built for";
-        return kEInval;
-      }
-      e.A = A;
-      e.B = B;
-      hipError_t he = hipMemcpyAsync(&g.d_ent[i], &e, sizeof(BatchEntry), hipMemcpyHostToDevice, b.stream);
-      if (he == hipSuccess) he = hipStreamSynchronize(b.stream);
-      if (he != hipSuccess) { err = hipGetErrorString(he); return kEHip; }
-      return kOk;
-    }
+  if (k >= b.item_group.size()) { err = "item index out of range"; return kEInval; }
+  BatchGroup &g = b.groups[b.item_group[k]];
+  const uint32_t i = b.item_pos[k];
+  BatchEntry &e = g.h_ent[i];
+  const uint32_t es = dtype_esize(g.dtype);
+  // Exact-vector sides were chosen for the old pointers' 16-byte alignment;
+  // new pointers must keep it there (unaligned sides accept any
+  // element-aligned pointer).
+  const bool vr = (e.variant & kVarVr) != 0, vw = (e.variant & kVarVw) != 0;
+  const bool rows = (e.variant & kVarRows) != 0;
+  const bool need_a = vr, need_b = rows ? vr : vw;
+  const uint32_t cap = 16 / es;
+  if ((need_a && align_vec(A, es, cap) != cap) || (need_b && align_vec(B, es, cap) != cap)) {
+    err = "new pointers are not 16-byte aligned like the ones the batch was built for";
+    return kEInval;
   }
-  err = "item index out of range";
-  return kEInval;
+  // Execs still running (on any stream) read the table: wait for the last
+  // one, then rewrite the entry synchronously.
+  hipError_t he = b.last_exec ? hipEventSynchronize(b.last_exec) : hipSuccess;
+  if (he == hipSuccess) {
+    e.A = A;
+    e.B = B;
+    he = hipMemcpy(&g.d_ent[i], &e, sizeof(BatchEntry), hipMemcpyHostToDevice);
+  }
+  if (he != hipSuccess) { err = hipGetErrorString(he); return kEHip; }
+  return kOk;
 }
 
 }  // namespace hpttx

@@ -107,6 +107,7 @@ int hpttx_batch_exec_on(hpttx_batch *h, void *stream) {
   } else {
     e = batch_enqueue(h->b, st);
   }
+  if (e == hipSuccess) e = hipEventRecord(h->b.last_exec, st);
   if (e != hipSuccess) {
     hpttx::set_last_error(std::string("batch launch: ") + hipGetErrorString(e));
     return HPTTX_EHIP;
@@ -159,6 +160,22 @@ int hpttx_batch_get_info(const hpttx_batch *h, hpttx_batch_info *info) {
   return HPTTX_OK;
 }
 
+int hpttx_batch_get_item_info(const hpttx_batch *h, uint32_t index, hpttx_batch_item_info *info) {
+  if (!h || !info || index >= h->b.item_group.size()) {
+    hpttx::set_last_error("bad batch, index or info pointer");
+    return HPTTX_EINVAL;
+  }
+  const BatchGroup &g = h->b.groups[h->b.item_group[index]];
+  const BatchEntry &e = g.h_ent[h->b.item_pos[index]];
+  std::memset(info, 0, sizeof(*info));
+  info->group = h->b.item_group[index];
+  info->kernel = g.variant;
+  info->variant = e.variant;
+  info->lds_bytes = e.lds_bytes;
+  info->tiles = e.ntiles;
+  return HPTTX_OK;
+}
+
 int hpttx_batch_item_path(const hpttx_batch *h, uint32_t index, char *buf, size_t len) {
   if (!h || index >= h->b.paths.size()) { hpttx::set_last_error("bad batch or index"); return HPTTX_EINVAL; }
   const std::string &s = h->b.paths[index];
@@ -172,6 +189,8 @@ int hpttx_batch_item_path(const hpttx_batch *h, uint32_t index, char *buf, size_
 
 void hpttx_batch_destroy(hpttx_batch *h) {
   if (!h) return;
+  // the last exec may have run on another stream than the batch's
+  if (h->b.last_exec) (void)hipEventSynchronize(h->b.last_exec);
   (void)hipStreamSynchronize(h->b.stream);
   batch_free(h->b);
   delete h;

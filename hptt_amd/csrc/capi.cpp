@@ -7,6 +7,7 @@
 #include <new>
 #include <string>
 
+#include "device.h"
 #include "hptt_amd.h"
 #include "kernels.h"
 #include "planner.h"
@@ -77,6 +78,14 @@ int create_common(int dtype, const void *in, void *out, uint32_t rank,
     delete p;
     return fail(rc, err);
   }
+  // Arguments first, then the device: the reference validates before it
+  // looks for a plugin that fits the host and returns nullptr when none does
+  // (src/hptt.cc:25-56, 86-87; src/arch/arch.cc:74-89).
+  rc = require_target_device(err);
+  if (rc != kOk) {
+    delete p;
+    return fail(rc, err);
+  }
   p->A = in;
   p->B = out;
   p->stream = static_cast<hipStream_t>(stream);
@@ -84,7 +93,8 @@ int create_common(int dtype, const void *in, void *out, uint32_t rank,
   p->coef = CoefD{ar, p->cc ? ai : 0.0, br, p->cc ? bi : 0.0};
   p->update = p->cc ? (std::fabs(br) >= 1e-16 || std::fabs(bi) >= 1e-16) : update_real(br);
   TuneParams tp{};
-  if (!tune_lookup(p->prob, p->update, tp) && tuning_timeout != 0.0) {
+  const bool cached = tune_lookup(p->prob, p->update, p->cc, tp);
+  if (!cached && tuning_timeout != 0.0) {
     rc = autotune(p->prob, p->coef, p->update, p->cc, tuning_timeout, p->stream, tp, err, in, out);
     if (rc != kOk) {
       delete p;
@@ -92,6 +102,12 @@ int create_common(int dtype, const void *in, void *out, uint32_t rank,
     }
   }
   rc = plan_launch(p->prob, tp, p->shape, err, kMinPlanTiles);
+  if (rc != kOk && cached) {
+    // A cached record that does not plan (from another build or a damaged
+    // file) is replaced by the model's pick rather than failing the plan.
+    tune_forget(tune_key(p->prob, p->update, p->cc));
+    rc = plan_launch(p->prob, TuneParams{}, p->shape, err, kMinPlanTiles);
+  }
   if (rc != kOk) {
     delete p;
     return fail(rc, err);

@@ -7,6 +7,8 @@
 #include <cstring>
 #include <sstream>
 
+#include "device.h"
+
 namespace hpttx {
 
 namespace {
@@ -73,6 +75,35 @@ void Arena::free(uint64_t off, uint64_t bytes) {
 }
 
 // ---------------------------------------------------------------------------
+uint64_t settle_requests(std::vector<Request *> &reqs, int status, const std::string &why,
+                         double t, ServerStats &st, std::vector<double> &lat,
+                         std::deque<Request *> &finished, Arena &arena) {
+  uint64_t bytes = 0;
+  for (Request *r : reqs) {
+    bytes += r->bytes;
+    r->t_done = t;
+    if (status == kOk) {
+      r->state = kDone;
+      r->status = kOk;
+      st.completed++;
+      st.bytes_done += r->bytes;
+      lat.push_back(t - r->t_submit);
+    } else {
+      r->state = kFailed;
+      r->status = status;
+      r->error = why;
+      st.failed++;
+      if (r->arena_out) arena.free(r->arena_off, r->arena_len);
+      r->arena_out = false;
+    }
+    finished.push_back(r);
+  }
+  if (status == kOk && !reqs.empty()) st.last_done = t;
+  reqs.clear();
+  return bytes;
+}
+
+// ---------------------------------------------------------------------------
 double Server::now() const {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - g_t0).count();
 }
@@ -83,6 +114,10 @@ int Server::start(const ServerConfig &cfg, std::string &err) {
   if (cfg_.slots > 8) cfg_.slots = 8;
   if (cfg_.max_batch < 1) cfg_.max_batch = 1;
   if (hipGetDevice(&device_) != hipSuccess) { err = "no HIP device"; return kEHip; }
+  {
+    int rc = require_target_device(err);
+    if (rc != kOk) return rc;
+  }
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) { err = "hipMemGetInfo failed"; return kEHip; }
   budget_ = cfg_.inflight_bytes ? cfg_.inflight_bytes : (uint64_t)fr / 4;
@@ -93,7 +128,9 @@ int Server::start(const ServerConfig &cfg, std::string &err) {
   slots_.resize(cfg_.slots);
   for (Slot &s : slots_) {
     hipError_t e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+    // Blocking-sync events: the completer can wait on one without spinning.
+    if (e == hipSuccess)
+      e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming | hipEventBlockingSync);
     if (e == hipSuccess) e = hipMalloc(&s.d_ent, sizeof(BatchEntry) * cfg_.max_batch);
     if (e == hipSuccess) e = hipMalloc(&s.d_begins, sizeof(uint32_t) * cfg_.max_batch);
     if (e == hipSuccess) e = hipHostMalloc(&s.h_ent, sizeof(BatchEntry) * cfg_.max_batch, hipHostMallocDefault);
@@ -113,25 +150,26 @@ int Server::start(const ServerConfig &cfg, std::string &err) {
 void Server::stop() {
   {
     std::unique_lock<std::mutex> lk(mu_);
-    // Requests still queued are failed; launches in flight finish first.
-    while (!pending_.empty()) {
-      Request *r = pending_.front();
-      pending_.pop_front();
-      r->state = kFailed;
-      r->status = kEInval;
-      finished_.push_back(r);
-    }
+    // From here on submit refuses; requests still queued fail (waiters are
+    // woken); launches in flight finish first.
+    stop_ = true;
+    std::vector<Request *> dropped(pending_.begin(), pending_.end());
+    pending_.clear();
+    settle_requests(dropped, kEInval, "server stopped before the request was admitted", now(),
+                    stats_, lat_, finished_, arena_);
+    progress_++;
+    cv_done_.notify_all();
     cv_done_.wait(lk, [&] {
-      for (const Slot &s : slots_)
-        if (s.busy) return false;
+      for (const Slot &s2 : slots_)
+        if (s2.busy) return false;
       return true;
     });
-    stop_ = true;
   }
   cv_dispatch_.notify_all();
   cv_complete_.notify_all();
   if (dispatcher_.joinable()) dispatcher_.join();
   if (completer_.joinable()) completer_.join();
+  std::lock_guard<std::mutex> lk(mu_);
   for (Slot &s : slots_) {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     if (s.d_ent) (void)hipFree(s.d_ent);
@@ -147,6 +185,7 @@ void Server::stop() {
   for (auto &kv : all_) delete kv.second;
   all_.clear();
   finished_.clear();
+  cv_done_.notify_all();
 }
 
 int Server::plan_cached(const BatchItemSpec &spec, PlannedItem &q, std::string &err) {
@@ -219,6 +258,19 @@ int Server::submit(const BatchItemSpec &spec_in, bool server_out, uint64_t &tick
   return kOk;
 }
 
+uint64_t Server::live_budget_locked() {
+  if (cfg_.inflight_bytes) return budget_ = cfg_.inflight_bytes;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget_ = (uint64_t)fr / 4;
+  else (void)hipGetLastError();   // keep the last budget
+  return budget_;
+}
+
+uint64_t Server::budget() {
+  std::lock_guard<std::mutex> lk(mu_);
+  return budget_;
+}
+
 void Server::dispatch_loop() {
   (void)hipSetDevice(device_);
   std::unique_lock<std::mutex> lk(mu_);
@@ -241,24 +293,26 @@ void Server::dispatch_loop() {
         slot = &s;
         break;
       }
-    // Admit in arrival order while the in-flight byte budget and the arena
-    // allow; a request is always admitted when nothing else is in flight.
+    // Admit in arrival order while the in-flight byte budget (from the HBM
+    // free now, so that allocations made outside the server shrink it) and
+    // the arena allow; a request is always admitted when nothing else is in
+    // flight.
+    const uint64_t budget = live_budget_locked();
     std::vector<Request *> batch;
     uint64_t add = 0;
     while (!pending_.empty() && batch.size() < cfg_.max_batch) {
       Request *r = pending_.front();
       const bool idle = inflight_ + add == 0;
-      if (!idle && inflight_ + add + r->bytes > budget_) break;
+      if (!idle && inflight_ + add + r->bytes > budget) break;
       if (r->arena_out) {
         uint64_t off = 0;
         if (!arena_.alloc(r->arena_len, off)) {
           if (idle && arena_.used() == 0) {
             pending_.pop_front();
-            r->state = kFailed;
-            r->status = kENoMem;
-            stats_.failed++;
+            std::vector<Request *> one{r};
+            settle_requests(one, kENoMem, "output does not fit the server's arena", now(), stats_,
+                            lat_, finished_, arena_);
             progress_++;
-            finished_.push_back(r);
             cv_done_.notify_all();
             continue;
           }
@@ -341,20 +395,13 @@ void Server::dispatch_loop() {
     if (he != hipSuccess) {
       // Nothing (more) was enqueued for this slot: fail its requests once the
       // stream has drained what was.
+      const std::string why = std::string("launch: ") + hipGetErrorString(he);
       lk.unlock();
       (void)hipStreamSynchronize(slot->stream);
       lk.lock();
-      for (Request *r : slot->reqs) {
-        r->state = kFailed;
-        r->status = kEHip;
-        stats_.failed++;
-        progress_++;
-        if (r->arena_out) arena_.free(r->arena_off, r->arena_len);
-        r->arena_out = false;
-        finished_.push_back(r);
-      }
+      settle_requests(slot->reqs, kEHip, why, now(), stats_, lat_, finished_, arena_);
+      progress_++;
       inflight_ -= slot->bytes;
-      slot->reqs.clear();
       slot->busy = false;
       cv_done_.notify_all();
       continue;
@@ -367,48 +414,68 @@ void Server::dispatch_loop() {
 
 void Server::complete_loop() {
   (void)hipSetDevice(device_);
+  // Poll the launched slots' events for up to kSpin (a timed sleep would add
+  // the kernel timer's slack, tens of microseconds, to every completion),
+  // then block on the oldest launch's event (blocking-sync: no host core is
+  // held while long launches run) and poll again.
+  constexpr double kSpin = 200e-6;
+  double spin_from = -1;
   for (;;) {
     std::vector<Slot *> live;
     {
       std::unique_lock<std::mutex> lk(mu_);
+      auto any_busy = [&] {
+        for (const Slot &s : slots_)
+          if (s.busy) return true;
+        return false;
+      };
       cv_complete_.wait(lk, [&] {
-        if (stop_) return true;
+        if (stop_ && !any_busy()) return true;
         for (const Slot &s : slots_)
           if (s.busy && (s.seq >> 63)) return true;
         return false;
       });
       for (Slot &s : slots_)
         if (s.busy && (s.seq >> 63)) live.push_back(&s);
-      if (stop_ && live.empty()) return;
+      if (live.empty()) {
+        if (stop_ && !any_busy()) return;
+        continue;
+      }
     }
-    std::vector<Slot *> done;
+    std::vector<std::pair<Slot *, hipError_t>> done;
     for (Slot *s : live) {
       const hipError_t e = hipEventQuery(s->done);
-      if (e == hipSuccess) done.push_back(s);
-      else if (e != hipErrorNotReady) done.push_back(s);   // surfaced at the next sync
+      if (e != hipErrorNotReady) done.push_back({s, e});
     }
     if (done.empty()) {
-      // Launches are in flight: keep polling (a timed sleep would add the
-      // kernel timer's slack, tens of microseconds, to every completion).
-      std::this_thread::yield();
+      const double t = now();
+      if (spin_from < 0) spin_from = t;
+      if (t - spin_from < kSpin) {
+        std::this_thread::yield();
+        continue;
+      }
+      Slot *oldest = live[0];
+      for (Slot *s : live)
+        if ((s->seq & ~(1ull << 63)) < (oldest->seq & ~(1ull << 63))) oldest = s;
+      (void)hipEventSynchronize(oldest->done);   // its status is read by the next query
       continue;
     }
+    spin_from = -1;
     const double t = now();
     {
       std::lock_guard<std::mutex> lk(mu_);
-      for (Slot *s : done) {
-        for (Request *r : s->reqs) {
-          r->t_done = t;
-          r->state = kDone;
-          stats_.completed++;
-          stats_.bytes_done += r->bytes;
-          lat_.push_back(t - r->t_submit);
-          finished_.push_back(r);
+      for (auto &sd : done) {
+        Slot *s = sd.first;
+        if (sd.second == hipSuccess) {
+          settle_requests(s->reqs, kOk, std::string(), t, stats_, lat_, finished_, arena_);
+        } else {
+          (void)hipGetLastError();
+          settle_requests(s->reqs, kEHip,
+                          std::string("launch failed: ") + hipGetErrorString(sd.second), t,
+                          stats_, lat_, finished_, arena_);
         }
-        stats_.last_done = t;
         progress_++;
         inflight_ -= s->bytes;
-        s->reqs.clear();
         s->busy = false;
         s->seq = 0;
       }

@@ -4,13 +4,15 @@
 // request is one transposition (the arguments of one plan creation plus its
 // data pointers).  Requests queue on the host; a dispatcher thread admits
 // them in arrival order while the bytes in flight stay within a budget taken
-// from free HBM and an output arena (for requests that let the server
-// allocate B) has room, packs every admitted request into one launch of the
-// batch kernel per element type, and issues it on the next free
-// stream slot -- so requests that arrive while others run are admitted as
-// slots and memory free up, and their launches overlap on the device.  A
-// completer thread polls each slot's event and stamps completion times.
-// Plans are cached by shape and descriptor tables stay resident per slot.
+// from the HBM free at that moment and an output arena (for requests that
+// let the server allocate B) has room, packs the admitted requests into
+// launches of the batch kernel (one per element type, kernel variant and LDS
+// class) and issues them on the next free stream slot -- so requests that
+// arrive while others run are admitted as slots and memory free up, and
+// their launches overlap on the device.  A completer thread polls each
+// slot's event for a short while, then blocks on the oldest one, and stamps
+// completion times.  Plans are cached by shape and descriptor tables stay
+// resident per slot.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,7 +36,8 @@ namespace hpttx {
 constexpr size_t kMaxSlotGroups = 512;
 
 struct ServerConfig {
-  uint64_t inflight_bytes = 0;   // budget of algorithmic bytes in flight (0: 1/4 of free HBM)
+  uint64_t inflight_bytes = 0;   // budget of algorithmic bytes in flight (0: a quarter of the
+                                 // HBM free at each admission round)
   uint64_t arena_bytes = 0;      // server-allocated outputs (0: none)
   uint32_t slots = 3;            // launches in flight at once (one stream each)
   uint32_t max_batch = 256;      // requests per launch
@@ -55,6 +58,7 @@ struct Request {
   uint32_t variant = 0;          // batch-kernel variant of its plan
   uint32_t lds_class = 0;        // LDS need in 8 KiB classes (launch grouping)
   double t_submit = 0, t_launch = 0, t_done = 0;
+  std::string error;             // why it failed (state kFailed)
 };
 
 // First-fit allocator over one device allocation (256-byte granules).
@@ -93,6 +97,17 @@ struct ServerStats {
   double first_submit = -1, last_done = -1;
 };
 
+// Completion bookkeeping shared by the completer (a slot's event reached or
+// failed) and the dispatcher (a launch that could not be enqueued), kept
+// free of HIP calls so that it can be tested on a host without a GPU.
+// status kOk marks every request done (latency recorded, bytes counted);
+// any other status marks it failed with that status and `why`.  Arena
+// outputs of failed requests are returned to `arena`.  Each request goes to
+// `finished`; the return value is the requests' algorithmic bytes.
+uint64_t settle_requests(std::vector<Request *> &reqs, int status, const std::string &why,
+                         double t, ServerStats &st, std::vector<double> &lat,
+                         std::deque<Request *> &finished, Arena &arena);
+
 class Server {
  public:
   int start(const ServerConfig &cfg, std::string &err);
@@ -106,13 +121,17 @@ class Server {
   ServerStats stats();
   void latencies(std::vector<double> &out);
   double now() const;
-  uint64_t budget() const { return budget_; }
+  // In-flight byte budget used by the last admission round.
+  uint64_t budget();
   char *arena_base() const { return arena_.base(); }
   const ServerConfig &config() const { return cfg_; }
 
  private:
   void dispatch_loop();
   void complete_loop();
+  // Budget for the next admission round (mu_ held): the configured one, or
+  // a quarter of the HBM free right now.
+  uint64_t live_budget_locked();
   int plan_cached(const BatchItemSpec &spec, PlannedItem &q, std::string &err);
 
   ServerConfig cfg_;

@@ -75,7 +75,11 @@ int hpttx_server_poll(hpttx_server *h, hpttx_completion *out, uint32_t max, uint
   if (!h || !n || (max && !out)) { set_last_error("null argument"); return HPTTX_EINVAL; }
   std::vector<Request> tmp(max);
   size_t got = h->s.poll(tmp.data(), max);
-  for (size_t i = 0; i < got; ++i) fill_completion(tmp[i], &out[i]);
+  for (size_t i = 0; i < got; ++i) {
+    fill_completion(tmp[i], &out[i]);
+    // a failed request's reason becomes this thread's last error
+    if (tmp[i].state == kFailed) set_last_error(tmp[i].error);
+  }
   *n = (uint32_t)got;
   return HPTTX_OK;
 }
@@ -88,6 +92,7 @@ int hpttx_server_wait(hpttx_server *h, uint64_t ticket, double timeout_s, hpttx_
     return HPTTX_EINVAL;
   }
   if (out) fill_completion(r, out);
+  if (r.state == kFailed) set_last_error(r.error);
   return HPTTX_OK;
 }
 

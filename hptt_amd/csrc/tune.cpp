@@ -10,6 +10,8 @@
 #include <mutex>
 #include <sstream>
 
+#include "device.h"
+
 namespace hpttx {
 
 namespace {
@@ -21,6 +23,14 @@ bool g_file_loaded = false;
 const char *cache_path() {
   const char *p = std::getenv("HPTTX_TUNE_CACHE");
   return (p && *p) ? p : nullptr;
+}
+
+// Field ranges a record must have to be used (anything else is a damaged or
+// foreign file; such lines are skipped).  0 means the model's value.
+bool plausible(const TuneParams &t) {
+  const uint32_t kMax = 1u << 20;
+  return t.run_r <= kMax && t.run_c <= kMax && t.block_elems <= kMax &&
+         t.wide_min_bytes <= kMax && t.stream <= 2;
 }
 
 void load_file_locked() {
@@ -35,8 +45,9 @@ void load_file_locked() {
     std::string key;
     TuneParams t;
     if (std::getline(is, key, '\t') &&
-        (is >> t.run_r >> t.run_c >> t.block_elems >> t.wide_min_bytes >> t.stream))
-      g_cache[key] = t;
+        (is >> t.run_r >> t.run_c >> t.block_elems >> t.wide_min_bytes >> t.stream) &&
+        plausible(t))
+      g_cache[key] = t;   // a later line for the same key wins
   }
 }
 
@@ -51,17 +62,20 @@ void store_locked(const std::string &key, const TuneParams &t) {
 
 }  // namespace
 
-std::string tune_key(const Problem &p, bool update) {
+std::string tune_key(const Problem &p, bool update, bool cc) {
+  std::string arch, err;
+  if (!device_arch(arch, err)) arch = "nodevice";
   std::ostringstream os;
-  os << "gfx950|" << p.dtype << '|' << (update ? 1 : 0) << '|' << p.fr;
+  os << arch << '|' << p.dtype << '|' << (update ? 1 : 0) << (cc ? "c" : "") << '|' << p.fr;
   for (int i = 0; i < p.fr; ++i) os << '|' << p.d[i].n << ':' << p.d[i].sa << ':' << p.d[i].sb;
   return os.str();
 }
 
-bool tune_lookup(const Problem &p, bool update, TuneParams &out) {
+bool tune_lookup(const Problem &p, bool update, bool cc, TuneParams &out) {
+  const std::string key = tune_key(p, update, cc);
   std::lock_guard<std::mutex> lk(g_mu);
   load_file_locked();
-  auto it = g_cache.find(tune_key(p, update));
+  auto it = g_cache.find(key);
   if (it == g_cache.end()) return false;
   out = it->second;
   return true;
@@ -83,6 +97,14 @@ bool tune_lookup_key(const std::string &key, TuneParams &out) {
   if (it == g_cache.end()) return false;
   out = it->second;
   return true;
+}
+
+void tune_forget(const std::string &key) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  load_file_locked();
+  // Appending the model's pick (all fields 0) overrides the bad line for
+  // every later load of the file.
+  store_locked(key, TuneParams{});
 }
 
 void tune_store_key(const std::string &key, const TuneParams &t) {
@@ -114,6 +136,7 @@ int autotune(const Problem &p, const CoefD &coef, bool update, bool cc, double t
              hipStream_t stream, TuneParams &out, std::string &err, const void *src_a,
              const void *src_b) {
   out = default_tune(p);
+  const std::string key = tune_key(p, update, cc);
   const size_t bytesA = p.spanA * p.esize, bytesB = p.spanB * p.esize;
   void *A = nullptr, *B = nullptr;
   if (hipMalloc(&A, bytesA) != hipSuccess) {
@@ -218,7 +241,7 @@ int autotune(const Problem &p, const CoefD &coef, bool update, bool cc, double t
   if (rc == kOk) {
     std::lock_guard<std::mutex> lk(g_mu);
     load_file_locked();
-    store_locked(tune_key(p, update), out);
+    store_locked(key, out);
   }
   return rc;
 }

@@ -19,12 +19,17 @@
 
 namespace hpttx {
 
-// Cache key: dtype, update flag and the fused problem (extents and strides).
-std::string tune_key(const Problem &p, bool update);
+// Cache key: the current device's architecture, dtype, update and
+// complex-coefficient flags and the fused problem (extents and strides).
+std::string tune_key(const Problem &p, bool update, bool cc);
 
 // Look the problem up in the cache (memory, then the file named by the
 // HPTTX_TUNE_CACHE environment variable).
-bool tune_lookup(const Problem &p, bool update, TuneParams &out);
+bool tune_lookup(const Problem &p, bool update, bool cc, TuneParams &out);
+
+// Replace a cached entry that no longer plans (a stale or foreign record)
+// with the model's pick, in memory and in the file.
+void tune_forget(const std::string &key);
 
 // Time candidate tile shapes and keep the fastest; records it in the cache.
 // The candidates run on scratch copies of the caller's A and B (src_a /

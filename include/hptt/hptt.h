@@ -19,7 +19,9 @@
 //     nothing); the out_outer_size check compares B's outer size with B's own
 //     extent (in_size[perm[d]]).
 // Added: overloads taking the stream, complex-coefficient overloads for the
-// complex types, sync(), describe(), native_handle().
+// complex types, sync(), describe(), native_handle(), status().  exec() does
+// not throw: a failed launch is written to stderr and reported by status()
+// and hpttx_last_error().
 //
 // Header-only over the C ABI in hptt_amd.h; link with -lhptt_amd.
 #pragma once
@@ -30,7 +32,6 @@
 #include <cstdio>
 #include <cstdint>
 #include <memory>
-#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -72,10 +73,12 @@ class CGraphTransPackBase {
   virtual void reset_data(const FloatType *in_data, FloatType *out_data) = 0;
   virtual void set_thread_ids(const std::vector<int32_t> &thread_ids) = 0;
   virtual void unset_thread_ids() = 0;
-  // Added: wait for the plan's stream; human-readable plan; C handle.
+  // Added: wait for the plan's stream; human-readable plan; C handle; the
+  // status (HPTTX_*) of the last exec / reset_data / sync.
   virtual void sync() = 0;
   virtual std::string describe() const = 0;
   virtual hpttx_plan *native_handle() const = 0;
+  virtual int status() const = 0;
 };
 
 template <typename FloatType>
@@ -83,27 +86,28 @@ using trans_plan = CGraphTransPackBase<FloatType>;
 
 namespace detail {
 
-inline void check(int rc) {
-  if (rc != HPTTX_OK) throw std::runtime_error(std::string("hptt_amd: ") + hpttx_last_error());
-}
-
 template <typename FloatType>
 class DevicePlan final : public CGraphTransPackBase<FloatType> {
  public:
   explicit DevicePlan(hpttx_plan *p) : p_(p) {}
   ~DevicePlan() override { hpttx_plan_destroy(p_); }
-  void exec() override { check(hpttx_plan_exec(p_)); }
-  void operator()() override { check(hpttx_plan_exec(p_)); }
+  // The reference's exec() / reset_data() return void and never fail
+  // (inc/hptt/impl/hptt_trans.h:27-30); a GPU launch can.  They stay void
+  // and do not throw: a failure is written to stderr, kept as status() and
+  // in hpttx_last_error().
+  void exec() override { report(hpttx_plan_exec(p_), "exec"); }
+  void operator()() override { exec(); }
   void print_plan() override {
     const std::string s = describe();
     std::fputs(s.c_str(), stdout);
   }
   void reset_data(const FloatType *in_data, FloatType *out_data) override {
-    check(hpttx_plan_reset_data(p_, in_data, out_data));
+    report(hpttx_plan_reset_data(p_, in_data, out_data), "reset_data");
   }
   void set_thread_ids(const std::vector<int32_t> &) override {}
   void unset_thread_ids() override {}
-  void sync() override { check(hpttx_plan_sync(p_)); }
+  void sync() override { report(hpttx_plan_sync(p_), "sync"); }
+  int status() const override { return status_; }
   std::string describe() const override {
     const int n = hpttx_plan_describe(p_, nullptr, 0);
     std::string s(n > 0 ? (size_t)n : 1, '\0');
@@ -114,7 +118,12 @@ class DevicePlan final : public CGraphTransPackBase<FloatType> {
   hpttx_plan *native_handle() const override { return p_; }
 
  private:
+  void report(int rc, const char *what) {
+    status_ = rc;
+    if (rc != HPTTX_OK) std::fprintf(stderr, "hptt_amd: %s failed: %s\n", what, hpttx_last_error());
+  }
   hpttx_plan *p_;
+  int status_ = HPTTX_OK;
 };
 
 inline const uint32_t *opt(const std::vector<uint32_t> &v) { return v.empty() ? nullptr : v.data(); }

@@ -204,6 +204,22 @@ typedef struct hpttx_batch_info {
   uint32_t lds_bytes[4];
 } hpttx_batch_info;
 int hpttx_batch_get_info(const hpttx_batch *batch, hpttx_batch_info *info);
+/* Where item `index` (creation order) runs: its launch group, the group's
+ * kernel (a variant code, or 128 for the instance that dispatches variants
+ * at run time), the item's own variant code (bits: 1 rows, 2 exact read
+ * vectors, 4 exact write vectors (rows: unaligned vectors), 8 update,
+ * 16 shared innermost group, 32 complex coefficients, 64 streaming), the LDS
+ * its tiles need and its tile count. */
+typedef struct hpttx_batch_item_info {
+  uint32_t group;
+  uint32_t kernel;
+  uint32_t variant;
+  uint32_t lds_bytes;
+  uint32_t tiles;
+  uint32_t reserved_;
+} hpttx_batch_item_info;
+int hpttx_batch_get_item_info(const hpttx_batch *batch, uint32_t index,
+                              hpttx_batch_item_info *info);
 /* Path of item `index` ("tile", "tile-shared", "wide-copy", "axpby",
  * "gather") into buf. */
 int hpttx_batch_item_path(const hpttx_batch *batch, uint32_t index, char *buf,
@@ -216,10 +232,11 @@ void hpttx_batch_destroy(hpttx_batch *batch);
  * inc/hptt/impl/hptt_trans.h:27).  Requests (hpttx_batch_item, one
  * transposition each) are queued by hpttx_server_submit and admitted in
  * arrival order while the algorithmic bytes in flight stay within the budget
- * (default: a quarter of the free HBM at creation) and the output arena has
- * room; every admitted request goes into the next launch of the persistent
- * batch kernel on one of `slots` streams, so requests arriving while others
- * run join the next launch and launches overlap on the device.  Plans are
+ * (default: a quarter of the HBM free at that admission round, so memory
+ * allocated outside the server shrinks it) and the output arena has room;
+ * the admitted requests go into the next launches of the batch kernels on
+ * one of `slots` streams, so requests arriving while others run join the
+ * next launch and launches overlap on the device.  Plans are
  * cached by shape.  A request whose `out` is NULL gets B allocated in the
  * server's arena (beta must then be below 1e-16); its address is reported at
  * completion and the buffer is held until hpttx_server_release.
@@ -227,7 +244,7 @@ void hpttx_batch_destroy(hpttx_batch *batch);
 typedef struct hpttx_server hpttx_server;
 
 typedef struct hpttx_server_config {
-  uint64_t inflight_bytes;   /* 0: a quarter of the free HBM */
+  uint64_t inflight_bytes;   /* 0: a quarter of the HBM free at each admission round */
   uint64_t arena_bytes;      /* bytes for server-allocated outputs (0: none) */
   uint32_t slots;            /* launches in flight (streams), 1..8; 0: 3 */
   uint32_t max_batch;        /* requests per launch; 0: 256 */
@@ -236,7 +253,8 @@ typedef struct hpttx_server_config {
 typedef struct hpttx_completion {
   uint64_t ticket;
   int32_t state;             /* 2 done, 3 failed */
-  int32_t status;            /* HPTTX_* when failed */
+  int32_t status;            /* HPTTX_* when failed; hpttx_server_poll / _wait then
+                                set hpttx_last_error() to the reason */
   void *out;                 /* B (the arena slot for server-allocated outputs) */
   double t_submit, t_launch, t_done;   /* seconds, one steady clock */
   uint64_t bytes;            /* algorithmic bytes */
@@ -247,7 +265,7 @@ typedef struct hpttx_server_stats {
   uint64_t bytes_done;
   double first_submit, last_done;      /* seconds, same clock as completions */
   double p50_latency, p99_latency, mean_latency;   /* submit -> done, seconds */
-  uint64_t inflight_budget;
+  uint64_t inflight_budget;  /* budget of the last admission round */
 } hpttx_server_stats;
 
 int hpttx_server_create(const hpttx_server_config *config, hpttx_server **server);

@@ -29,8 +29,13 @@ class HpttxPlan final : public hptt::CGraphTransPackBase<T> {
  public:
   explicit HpttxPlan(hpttx_plan *p) : p_(p) {}
   ~HpttxPlan() override { hpttx_plan_destroy(p_); }
+  // The reference's exec() returns void and cannot report failure
+  // (inc/hptt/impl/hptt_trans.h:27): a failed launch or wait is written to
+  // stderr and stays readable through hpttx_last_error().
   void exec() override {
-    if (hpttx_plan_exec(p_) == HPTTX_OK) (void)hpttx_plan_sync(p_);
+    int rc = hpttx_plan_exec(p_);
+    if (rc == HPTTX_OK) rc = hpttx_plan_sync(p_);
+    if (rc != HPTTX_OK) std::fprintf(stderr, "hptt plugin: exec failed: %s\n", hpttx_last_error());
   }
   void operator()() override { exec(); }
   void print_plan() override {
@@ -39,7 +44,8 @@ class HpttxPlan final : public hptt::CGraphTransPackBase<T> {
     std::fputs(buf, stdout);
   }
   void reset_data(const T *in_data, T *out_data) override {
-    (void)hpttx_plan_reset_data(p_, in_data, out_data);
+    if (hpttx_plan_reset_data(p_, in_data, out_data) != HPTTX_OK)
+      std::fprintf(stderr, "hptt plugin: reset_data failed: %s\n", hpttx_last_error());
   }
   void set_thread_ids(const std::vector<int32_t> &) override {}
   void unset_thread_ids() override {}

@@ -77,6 +77,7 @@ def test_python_structs_match_the_c_abi(tmp_path):
     compiler), so the binding cannot read or write past a field."""
     import hptt_amd
     pairs = [(hptt_amd.PlanInfo, "hpttx_plan_info"), (hptt_amd.BatchItem, "hpttx_batch_item"),
+             (hptt_amd.BatchItemInfo, "hpttx_batch_item_info"),
              (hptt_amd.BatchInfo, "hpttx_batch_info"),
              (hptt_amd.ServerConfigC, "hpttx_server_config"),
              (hptt_amd.Completion, "hpttx_completion"), (hptt_amd.ServerStats, "hpttx_server_stats")]
@@ -159,3 +160,44 @@ def test_reference_plugin_shim_exports_the_factories(tmp_path):
     exported = {line.split()[-1] for line in nm.stdout.splitlines() if line.strip()}
     for name in REFERENCE_FACTORIES:
         assert name in exported, name
+
+
+def test_python_shape_checks_before_the_c_abi():
+    """The binding refuses arrays of the wrong length before the C ABI reads
+    rank entries of each (the reference returns nullptr, src/hptt.cc:35-39),
+    and computes the spans tensors must hold."""
+    import hptt_amd
+    fake = 0x100000
+    bad = [
+        dict(in_size=[4, 4, 4], perm=[1, 0]),
+        dict(in_size=[4], perm=[1, 0]),
+        dict(in_size=[4, 4], perm=[1, 0], in_outer_size=[5]),
+        dict(in_size=[4, 4], perm=[1, 0], out_outer_size=[5, 5, 5]),
+    ]
+    for kw in bad:
+        assert hptt_amd.create_plan(fake, fake, alpha=1.0, beta=0.0, dtype="s", **kw) is None
+        with pytest.raises(hptt_amd.HpttError) as ei:
+            hptt_amd.create_plan_checked(fake, fake, alpha=1.0, beta=0.0, dtype="s", **kw)
+        assert ei.value.status == hptt_amd.EINVAL
+        with pytest.raises(hptt_amd.HpttError):
+            hptt_amd.make_batch_item(fake, fake, kw["in_size"], kw["perm"], 1.0, 0.0, dtype="s",
+                                     in_outer_size=kw.get("in_outer_size", ()),
+                                     out_outer_size=kw.get("out_outer_size", ()))
+    with pytest.raises(hptt_amd.HpttError) as ei:
+        hptt_amd.make_batch_item(fake, fake, [2] * 7, list(range(7)), 1.0, 0.0, dtype="s")
+    assert ei.value.status == hptt_amd.EUNSUPPORTED
+    # spans: column-major over the outer sizes (inc/hptt/tensor.tcc:186-196)
+    assert hptt_amd._spans([3, 4], [1, 0], (), ()) == [12, 12]
+    assert hptt_amd._spans([3, 4], [1, 0], [5, 4], [6, 3]) == [1 + 2 + 3 * 5, 1 + 3 + 2 * 6]
+    assert hptt_amd._spans([3, 4], [0, 0], (), ()) is None
+
+
+def test_server_completion_bookkeeping_host():
+    """tests/cpp/serve_book_test: a failed slot's requests are marked failed
+    with the reason and counted, a completed slot's done (no GPU used)."""
+    exe = os.path.join(ROOT, "tests", "cpp", "serve_book_test")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "cpp"),
+                        "serve_book_test"], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "SERVE BOOK OK" in r.stdout, r.stdout + r.stderr

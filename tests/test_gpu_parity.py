@@ -308,3 +308,29 @@ def test_concurrent_exec_from_host_threads(device):
     assert not errors, errors
     for k in [0] + list(range(2, 8)):
         assert oracle.first_mismatch(host(outs[k]), exp) == -1, k
+
+
+def test_bad_tuning_cache_record_falls_back(device, tmp_path):
+    """A tuning-cache record that does not plan (here tile runs of 4096
+    elements, which need far more than 64 KiB of LDS) is replaced by the
+    model's pick instead of failing create_plan; the plan is exact.  Runs in
+    a fresh process, which reads the cache file named by HPTTX_TUNE_CACHE."""
+    import subprocess
+    import sys
+    root = os.path.dirname(HERE)
+    cache = tmp_path / "tune.txt"
+    key = "gfx950|0|0|2|8192:1:8192|8192:8192:1"
+    cache.write_text(key + "\t4096 4096 1048576 256 0\n")
+    script = (
+        "import torch, hptt_amd\n"
+        "A = torch.rand(8192 * 8192, device='cuda')\n"
+        "B = torch.zeros_like(A)\n"
+        "p = hptt_amd.create_plan_checked(A, B, [8192, 8192], [1, 0], 1.0, 0.0)\n"
+        "p.exec(); p.sync()\n"
+        "assert torch.equal(B.view(8192, 8192), A.view(8192, 8192).t())\n"
+        "print('FALLBACK OK')\n")
+    env = dict(os.environ, HPTTX_TUNE_CACHE=str(cache))
+    r = subprocess.run([sys.executable, "-c", script], cwd=root, env=env, capture_output=True,
+                       text=True, timeout=110)
+    assert r.returncode == 0 and "FALLBACK OK" in r.stdout, r.stdout + r.stderr
+    assert cache.read_text().splitlines()[-1] == key + "\t0 0 0 0 0"

@@ -130,3 +130,106 @@ def test_admission_budget_serialises(device):
         assert srv.wait(t, 5.0).state == 2
         assert oracle.first_mismatch(host(r["dB"]), _alone(r)) == -1
     srv.close()
+
+
+# One request of at least 16 Mi elements per element type (shape, perm).
+BIG = {0: ([256, 256, 256], [2, 0, 1]),
+       1: ([64, 64, 64, 64], [3, 1, 0, 2]),
+       2: ([4096, 4096], [1, 0]),
+       3: ([32, 16, 32, 16, 64], [4, 2, 0, 3, 1])}
+
+
+def _poisson_requests(rng, count):
+    reqs = []
+    big_at = set(int(v) for v in rng.choice(np.arange(4, count), size=4, replace=False))
+    codes = iter(range(4))
+    for k in range(count):
+        if k in big_at:
+            code = next(codes)
+            size, perm = BIG[code]
+        else:
+            code = int(rng.integers(0, 4))
+            n_target = float(np.exp(rng.uniform(np.log(1024), np.log(2 * 2**20))))
+            size, perm = shapes.random_case(rng, max_elems=int(n_target), max_dim=128)
+        cc = code >= 2 and k % 3 == 0
+        al = complex(rng.uniform(-2, 2), rng.uniform(-1, 1)) if cc else float(rng.uniform(-2, 2))
+        if k % 4 == 0:
+            be = 0.0
+        else:
+            be = complex(rng.uniform(-2, 2), rng.uniform(-1, 1)) if cc else float(rng.uniform(-2, 2))
+        n = int(np.prod(size))
+        A = oracle.uniform(n, NP[code], 50 * k + 1, nthreads=16)
+        B = oracle.uniform(n, NP[code], 50 * k + 2, nthreads=16)
+        reqs.append(dict(code=code, size=size, perm=perm, alpha=al, beta=be, cc=cc, A=A, B=B,
+                         dA=dev(A), dB=dev(B)))
+    return reqs
+
+
+def test_poisson_stream_with_large_requests(device):
+    """A submitter thread issues requests at Poisson arrival times (mean gap
+    0.2 ms), among them one of >= 16 Mi elements per element type, while
+    earlier requests run; every result equals the oracle and the same
+    transposition run alone through a plain plan."""
+    import torch
+    rng = np.random.default_rng(4242)
+    reqs = _poisson_requests(rng, 48)
+    assert sum(int(np.prod(r["size"])) >= 16 * 2**20 for r in reqs) == 4
+    torch.cuda.synchronize()
+    gaps = rng.exponential(0.2e-3, size=len(reqs))
+    srv = hptt_amd.Server(slots=3, max_batch=64)
+    tickets = [None] * len(reqs)
+
+    def client():
+        t0 = time.perf_counter()
+        due = 0.0
+        for k, r in enumerate(reqs):
+            due += gaps[k]
+            while time.perf_counter() - t0 < due:
+                pass
+            tickets[k] = srv.submit(_item(r))
+
+    th = threading.Thread(target=client)
+    th.start()
+    th.join()
+    srv.drain()
+    st = srv.stats()
+    assert st.completed == len(reqs) and st.failed == 0
+    print("poisson serving: %d requests in %d launches" % (len(reqs), st.launches), flush=True)
+    for k, r in enumerate(reqs):
+        c = srv.wait(tickets[k], 10.0)
+        assert c.state == 2, (k, c.status)
+        got = host(r["dB"])
+        exp = r["B"].copy()
+        assert oracle.transpose(r["A"], exp, r["size"], r["perm"], r["alpha"], r["beta"],
+                                nthreads=16, complex_coef=r["cc"]) == 0
+        assert oracle.first_mismatch(got, exp) == -1, (k, r["size"], r["perm"])
+        assert oracle.first_mismatch(got, _alone(r)) == -1, k
+    srv.close()
+
+
+def test_admission_follows_free_hbm(device):
+    """With the default budget (a quarter of the HBM free at each admission
+    round), an allocation made outside the server shrinks the budget the next
+    admission uses by about a quarter of its size, and freeing it restores
+    the budget."""
+    import torch
+    rng = np.random.default_rng(6)
+    reqs = _requests(rng, 3, max_elems=20000)
+    torch.cuda.synchronize()
+    srv = hptt_amd.Server(slots=2)
+
+    def admitted_budget(r):
+        srv.wait(srv.submit(_item(r)), 10.0)
+        return srv.stats().inflight_budget
+
+    b0 = admitted_budget(reqs[0])
+    hog_bytes = 64 << 30
+    hog = torch.empty(hog_bytes // 4, dtype=torch.float32, device="cuda")
+    b1 = admitted_budget(reqs[1])
+    del hog
+    torch.cuda.empty_cache()
+    b2 = admitted_budget(reqs[2])
+    srv.close()
+    drop = b0 - b1
+    assert 0.8 * hog_bytes / 4 <= drop <= 1.2 * hog_bytes / 4, (b0, b1, drop)
+    assert b2 > b1 + 0.8 * hog_bytes / 4, (b1, b2)
