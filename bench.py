@@ -151,7 +151,7 @@ def time_workload(name, steps, warmup, tune=0.0):
     return res
 
 
-def time_batched(dtype_name, reps=5, seq_reps=3, tune=0.0):
+def time_batched(dtype_name, reps=5, seq_reps=3, tune=0.0, one_launch=False):
     """The batched workload: 4096 tensors (seeded manifest, rank 2..6, dims
     log-uniform in [2,128], <= 4 Mi elements each, random perms, 25% beta=0)
     as one Batch (work-queue launches, graph replay) against one plan->exec()
@@ -182,7 +182,7 @@ def time_batched(dtype_name, reps=5, seq_reps=3, tune=0.0):
         plans.append(hptt_amd.create_plan_checked(a, b, m["in_size"], m["perm"], m["alpha"],
                                                   m["beta"], stream=stream))
     t_build = time.perf_counter()
-    batch = hptt_amd.Batch(items, stream=stream, tuning_timeout=tune)
+    batch = hptt_amd.Batch(items, stream=stream, tuning_timeout=tune, one_launch=one_launch)
     t_build = time.perf_counter() - t_build
     info = batch.info()
     exec_fn = hptt_amd.lib().hpttx_plan_exec
@@ -235,6 +235,8 @@ def time_batched(dtype_name, reps=5, seq_reps=3, tune=0.0):
            "algorithmic_bytes": info.algorithmic_bytes, "tiles": info.tiles,
            "grid": list(info.grid)[:info.groups], "build_s": round(t_build, 3),
            "tuning_timeout": tune, "groups": info.groups,
+           "layout": "one persistent launch per element type over a dynamic queue" if one_launch
+                     else "one launch per (element type, variant, LDS class), one block per tile",
            "identical_to_sequential": same}
     del batch, plans, A, B, B0, Bseq
     torch.cuda.empty_cache()
@@ -550,7 +552,8 @@ def main():
     ap.add_argument("--tune", type=float, default=0.0,
                     help="autotune each plan for this many seconds before timing")
     ap.add_argument("--workload", default="6d-f32",
-                    help="main workload (%s) or batched-f32 / batched-f64" % ", ".join(WORKLOADS))
+                    help="main workload (%s), batched-f32 / batched-f64 (suffix -one-launch "
+                         "for that layout), serving or sweep" % ", ".join(WORKLOADS))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -570,8 +573,9 @@ def main():
         print(json.dumps({"workload": "serving", **res}), flush=True)
         return
     if args.workload.startswith("batched-"):
-        res = time_batched({"batched-f32": "float32", "batched-f64": "float64"}[args.workload],
-                           reps=max(1, args.steps), tune=args.tune)
+        res = time_batched({"f32": "float32", "f64": "float64"}[args.workload.split("-")[1]],
+                           reps=max(1, args.steps), tune=args.tune,
+                           one_launch=args.workload.endswith("one-launch"))
         print(json.dumps({"workload": args.workload, **res}), flush=True)
         return
     log("bench: main workload %s, %d steps" % (args.workload, args.steps))
@@ -592,8 +596,10 @@ def main():
         sw = secondary["reference-sweep"]
         log("bench: reference sweep (%d shapes) min %.1f median %.1f GB/s; slowest %s"
             % (sw["shapes"], sw["min_gbps"], sw["median_gbps"], sw["slowest"][0]["name"]))
-        for name in ("batched-f32", "batched-f64"):
-            secondary[name] = time_batched(name.split("-")[1].replace("f", "float"))
+        for name in ("batched-f32", "batched-f64", "batched-f32-one-launch",
+                     "batched-f64-one-launch"):
+            secondary[name] = time_batched(name.split("-")[1].replace("f", "float"),
+                                           one_launch=name.endswith("one-launch"))
             log("bench: %s %s transposes/s batched vs %s sequential (x%.2f), identical=%s"
                 % (name, secondary[name]["batched_transposes_per_s"],
                    secondary[name]["sequential_transposes_per_s"], secondary[name]["speedup"],

@@ -31,6 +31,7 @@ OK, EINVAL, ENOMEM, EHIP, EUNSUPPORTED = 0, 1, 2, 3, 4
 F32, F64, C64, C128 = 0, 1, 2, 3
 ELEM_BYTES = {F32: 4, F64: 8, C64: 8, C128: 16}
 PATHS = {0: "tile", 1: "tile-shared", 2: "wide-copy", 3: "axpby", 4: "gather"}
+BATCH_ONE_LAUNCH = 1
 
 _lib = None
 
@@ -338,6 +339,9 @@ def _bind_batch(L):
     L.hpttx_batch_create.restype = ctypes.c_int
     L.hpttx_batch_create.argtypes = [u32, ctypes.POINTER(BatchItem), d, vp,
                                      ctypes.POINTER(ctypes.c_void_p)]
+    L.hpttx_batch_create_ex.restype = ctypes.c_int
+    L.hpttx_batch_create_ex.argtypes = [u32, ctypes.POINTER(BatchItem), d, vp, u32,
+                                        ctypes.POINTER(ctypes.c_void_p)]
     for name in ("hpttx_batch_exec", "hpttx_batch_sync"):
         getattr(L, name).restype = ctypes.c_int
         getattr(L, name).argtypes = [vp]
@@ -397,15 +401,18 @@ def make_batch_item(in_data, out_data, in_size, perm, alpha, beta, dtype=None,
 class Batch:
     """Many transpositions executed by work-queue launches over one tile
     queue (one launch per element type, kernel variant and LDS class, forked
-    over side streams), replayed from a captured hipGraph (hpttx_batch_* in
-    include/hptt_amd.h).  items: a sequence of BatchItem."""
+    over side streams; or, with one_launch, one persistent launch per element
+    type over a dynamic queue), replayed from a captured hipGraph
+    (hpttx_batch_* in include/hptt_amd.h).  items: a sequence of BatchItem."""
 
-    def __init__(self, items, tuning_timeout: float = 0.0, stream=None, keep=()):
+    def __init__(self, items, tuning_timeout: float = 0.0, stream=None, keep=(),
+                 one_launch: bool = False):
         L = lib()
         arr = (BatchItem * len(items))(*items)
         h = ctypes.c_void_p()
-        _check(L.hpttx_batch_create(len(items), arr, float(tuning_timeout),
-                                    _stream_ptr(stream), ctypes.byref(h)))
+        flags = BATCH_ONE_LAUNCH if one_launch else 0
+        _check(L.hpttx_batch_create_ex(len(items), arr, float(tuning_timeout),
+                                       _stream_ptr(stream), flags, ctypes.byref(h)))
         self._h = h
         self.count = len(items)
         self._keep = list(keep)

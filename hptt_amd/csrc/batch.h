@@ -72,26 +72,43 @@ void make_entry(const PlannedItem &q, const BatchItemSpec &it, BatchEntry &e);
 uint32_t batch_lds_bytes(uint32_t lds_bytes);
 // LDS class of an entry (8 KiB steps): split-mode launches group by it.
 uint32_t batch_lds_class(uint32_t lds_bytes);
-// Blocks of one launch.  The kernel is a work-queue walker over the
-// launch's tiles (block b takes slots b, b + grid, ...).  By default it is
-// launched with one block per tile, so the hardware dispatcher hands out the
-// queue in order as blocks retire: measured faster on MI355X than a
-// persistent grid, whose blocks drift apart and lose the in-order sweep
-// (fp64 manifest of 4096 tensors: 9.2 ms against 10.6 ms).  With
-// HPTTX_BATCH_GRID=persistent the grid is the resident-block count
-// (occupancy of the (dtype, variant) kernel at lds_bytes, times the CUs;
-// looked up once and cached).
+// Blocks of one launch.  The kernel walks the launch's tiles either by a
+// static stride (block b takes slots b, b + grid, ...) or, with a dynamic
+// queue, by taking chunks off a device counter.  By default it is launched
+// with one block per tile, so the hardware dispatcher hands out the tiles in
+// order as blocks retire: measured faster on MI355X than a persistent grid
+// walking a static stride, whose blocks drift apart and lose the in-order
+// sweep (fp64 manifest of 4096 tensors: 9.2 ms against 10.6 ms).  With
+// `persistent` the grid is the resident-block count (occupancy of the
+// (dtype, variant) kernel at lds_bytes, times the CUs; looked up once and
+// cached).  HPTTX_BATCH_GRID=persistent makes batch_persistent() true.
 bool batch_persistent();
-uint32_t batch_grid(int dtype, uint32_t variant, uint32_t lds_bytes, uint32_t total_tiles);
+uint32_t batch_grid(int dtype, uint32_t variant, uint32_t lds_bytes, uint32_t total_tiles,
+                    bool persistent);
+// A dynamic work queue over one launch's tiles: the tiles (largest
+// transposition first) cut into chunks that shrink towards the end (each a
+// quarter of the remaining tiles' share per block, at least one tile), and a
+// device counter from which every block takes the index of its next chunk
+// with one atomic increment.  A kernel node clears the counter before each
+// launch (no memset node, so the sequence can be captured and replayed).
+struct BatchQueue {
+  uint32_t *counter = nullptr;        // device, one uint32
+  const uint32_t *chunks = nullptr;   // device, nchunks + 1 chunk boundaries
+  uint32_t nchunks = 0;
+};
+std::vector<uint32_t> batch_queue_chunks(uint32_t total_tiles, uint32_t grid);
 // Launch the batch kernel over entries [0, nent) whose tiles number
 // `total`, on `grid` blocks (a 2-D grid beyond 2^20 blocks).  `variant` is
 // a variant code (every entry has it) or batch_group_key(., true) for the
 // instance that dispatches per entry.
 // d_ent_of: null (each block binary-searches d_begins for its tile's entry)
 // or the entry of every tile (batch_entry_of_tile: one load per block).
+// queue: null (static stride) or a dynamic queue; its counter is cleared
+// by a kernel launched just before, on the same stream.
 hipError_t launch_batch_group(int dtype, uint32_t variant, const BatchEntry *d_ent,
                               const uint32_t *d_begins, const uint32_t *d_ent_of, uint32_t nent,
-                              uint32_t total, uint32_t lds_bytes, uint32_t grid, hipStream_t st);
+                              uint32_t total, uint32_t lds_bytes, uint32_t grid, hipStream_t st,
+                              const BatchQueue *queue = nullptr);
 // Entry index of each of `total` tiles for entries whose first tiles are
 // `begins`.
 std::vector<uint32_t> batch_entry_of_tile(const std::vector<uint32_t> &begins, uint32_t total);
@@ -105,6 +122,14 @@ uint32_t batch_group_key(uint32_t variant, bool merged);
 // instance holds only its own variant's registers).
 bool batch_merged_default(bool fallback);
 
+// Batch layouts (hpttx_batch_create_ex flags).
+//   split (default): one launch per (element type, variant, LDS class), one
+//     block per tile, forked over side streams.
+//   one launch: one persistent launch per element type whose blocks take
+//     chunks of tiles off a dynamic queue (the kernel dispatches each
+//     entry's variant at run time).
+constexpr uint32_t kBatchOneLaunch = 1;
+
 // The entries of one launch: one element type, and one variant unless merged.
 struct BatchGroup {
   int dtype = 0;
@@ -116,6 +141,9 @@ struct BatchGroup {
   BatchEntry *d_ent = nullptr;      // device table
   uint32_t *d_begins = nullptr;     // device: first tile of each entry
   uint32_t *d_ent_of = nullptr;     // device: entry of each tile
+  uint32_t *d_queue = nullptr;      // device: dynamic-queue counter (one-launch layout)
+  uint32_t *d_chunks = nullptr;     // device: its chunk boundaries
+  uint32_t nchunks = 0;             // 0: static walk
   std::vector<BatchEntry> h_ent;    // host copy
   std::vector<uint32_t> order;      // entry k <- user item order[k]
   uint64_t bytes = 0;               // algorithmic bytes
@@ -142,9 +170,10 @@ struct Batch {
   std::vector<uint32_t> item_pos;   // per user item: its entry within the group
 };
 
-// Plan every item (tile shapes tuned per class when tuning_timeout != 0).
+// Plan every item (tile shapes tuned per class when tuning_timeout != 0)
+// and lay the launches out as `flags` asks (kBatchOneLaunch or 0).
 int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, double tuning_timeout,
-                Batch &b, std::string &err);
+                uint32_t flags, Batch &b, std::string &err);
 // Enqueue one kernel launch per group, forked over the side streams.
 hipError_t batch_enqueue(Batch &b, hipStream_t stream);
 // Capture batch_enqueue into a graph (done on first exec).

@@ -119,14 +119,36 @@ uint32_t batch_lds_bytes(uint32_t lds_bytes) { return (lds_bytes + 15u) & ~15u; 
 
 uint32_t batch_lds_class(uint32_t lds_bytes) { return (lds_bytes + 8191u) / 8192u; }
 
+// Clears a dynamic queue's counter (a kernel node, so that a captured graph
+// replays it; atomic, so the next launch's atomic increments on every XCD
+// see it).
+__global__ void k_queue_reset(uint32_t *counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) atomicExch(counter, 0u);
+}
+
+std::vector<uint32_t> batch_queue_chunks(uint32_t total, uint32_t grid) {
+  std::vector<uint32_t> c;
+  const uint64_t div = 4ull * std::max<uint32_t>(grid, 1);
+  uint32_t t = 0;
+  while (t < total) {
+    c.push_back(t);
+    const uint32_t rem = total - t;
+    const uint32_t n = (uint32_t)std::max<uint64_t>(1, rem / div);
+    t += std::min(n, rem);
+  }
+  c.push_back(total);
+  return c;
+}
+
 bool batch_persistent() {
   const char *m = std::getenv("HPTTX_BATCH_GRID");
   return m && std::strcmp(m, "persistent") == 0;
 }
 
-uint32_t batch_grid(int dtype, uint32_t variant, uint32_t lds_bytes, uint32_t total_tiles) {
+uint32_t batch_grid(int dtype, uint32_t variant, uint32_t lds_bytes, uint32_t total_tiles,
+                    bool persistent) {
   total_tiles = std::max<uint32_t>(1, total_tiles);
-  if (!batch_persistent()) return total_tiles;
+  if (!persistent) return total_tiles;
   static std::mutex mu;
   static std::map<std::tuple<int, uint32_t, uint32_t>, uint32_t> cache;
   static int ncu = 0;
@@ -171,12 +193,21 @@ std::vector<uint32_t> batch_entry_of_tile(const std::vector<uint32_t> &begins, u
 
 hipError_t launch_batch_group(int dtype, uint32_t variant, const BatchEntry *d_ent,
                               const uint32_t *d_begins, const uint32_t *d_ent_of, uint32_t nent,
-                              uint32_t total, uint32_t lds_bytes, uint32_t grid, hipStream_t st) {
+                              uint32_t total, uint32_t lds_bytes, uint32_t grid, hipStream_t st,
+                              const BatchQueue *queue) {
   if (!total || !nent) return hipSuccess;
   const void *k = batch_kernel(dtype, variant);
   if (!k) return hipErrorInvalidValue;
+  uint32_t *counter = queue ? queue->counter : nullptr;
+  const uint32_t *chunks = queue ? queue->chunks : nullptr;
+  uint32_t nchunks = queue ? queue->nchunks : 0;
+  if (counter) {
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, counter);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   void *args[] = {(void *)&d_ent, (void *)&d_begins, (void *)&d_ent_of, (void *)&nent,
-                  (void *)&total};
+                  (void *)&total, (void *)&counter, (void *)&chunks, (void *)&nchunks};
   // 2-D grid: x * 256 threads must stay below 2^32 work items per dimension.
   const uint32_t gx = grid < kGridX ? grid : kGridX;
   return hipLaunchKernel(k, dim3(gx, (grid + gx - 1) / gx), dim3(kBlock), args, lds_bytes, st);
@@ -278,7 +309,7 @@ int tune_class_batch(const PlannedItem &rep, const BatchItemSpec &spec, double b
         hipMemcpyAsync(d_begins, c.begins.data(), sizeof(uint32_t) * K, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(d_ent_of, ent_of.data(), sizeof(uint32_t) * ent_of.size(), hipMemcpyHostToDevice, st) != hipSuccess)
       return false;
-    const uint32_t grid = batch_grid(spec.dtype, c.key, c.lds, c.total);
+    const uint32_t grid = batch_grid(spec.dtype, c.key, c.lds, c.total, batch_persistent());
     if (launch_batch_group(spec.dtype, c.key, d_ent, d_begins, d_ent_of, (uint32_t)K, c.total, c.lds, grid, st) != hipSuccess)
       return false;
     (void)hipEventRecord(e0, st);
@@ -351,7 +382,7 @@ int tune_class_batch(const PlannedItem &rep, const BatchItemSpec &spec, double b
 }  // namespace
 
 int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, double tuning_timeout,
-                Batch &b, std::string &err) {
+                uint32_t flags, Batch &b, std::string &err) {
   b = Batch();
   b.count = (uint32_t)items.size();
   b.stream = stream;
@@ -402,12 +433,21 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
   // Batches: one launch per (element type, variant), forked over streams
   // (measured faster than one dispatching launch per type on the 4096-tensor
   // mix: 4.93 ms vs 5.53 ms fp32, 8.71 vs 10.18 ms fp64).
-  const bool merged = batch_merged_default(false);
+  // The one-launch layout: one persistent launch per element type over a
+  // dynamic queue.  Otherwise the environment may still ask for merged
+  // launches (HPTTX_BATCH_MODE=merged), persistent grids
+  // (HPTTX_BATCH_GRID=persistent) or the dynamic queue (HPTTX_BATCH_GRID=
+  // queue, persistent grids with a dynamic queue).
+  const char *grid_env = std::getenv("HPTTX_BATCH_GRID");
+  const bool env_queue = grid_env && std::strcmp(grid_env, "queue") == 0;
+  const bool one_launch = (flags & kBatchOneLaunch) != 0;
+  const bool merged = one_launch || batch_merged_default(false);
+  const bool use_queue = one_launch || env_queue;
+  const bool persistent = use_queue || batch_persistent();
   // Groups: (element type, kernel, LDS class).  A launch reserves its
   // largest entry's LDS for every block, so entries are also split by LDS
   // need (8 KiB classes): small tiles keep their occupancy.  Merged mode
-  // keeps one launch per element type (with HPTTX_BATCH_GRID=persistent:
-  // one persistent work-queue launch per type).
+  // keeps one launch per element type.
   std::map<std::tuple<int, uint32_t, uint32_t>, std::vector<size_t>> by_kernel;
   for (size_t k = 0; k < items.size(); ++k) {
     b.paths[k] = pl[k].lp.path;
@@ -445,9 +485,19 @@ int batch_build(const std::vector<BatchItemSpec> &items, hipStream_t stream, dou
     g.nent = (uint32_t)g.h_ent.size();
     g.total_tiles = (uint32_t)tiles;
     g.lds_bytes = batch_lds_bytes(g.lds_bytes);
-    g.grid = batch_grid(g.dtype, g.variant, g.lds_bytes, g.total_tiles);
+    g.grid = batch_grid(g.dtype, g.variant, g.lds_bytes, g.total_tiles, persistent);
     const std::vector<uint32_t> ent_of = batch_entry_of_tile(begins, g.total_tiles);
-    he = hipMalloc(&g.d_ent, sizeof(BatchEntry) * g.nent);
+    std::vector<uint32_t> chunks;
+    if (use_queue) {
+      chunks = batch_queue_chunks(g.total_tiles, g.grid);
+      g.nchunks = (uint32_t)chunks.size() - 1;
+      he = hipMalloc(&g.d_queue, sizeof(uint32_t));
+      if (he == hipSuccess) he = hipMalloc(&g.d_chunks, sizeof(uint32_t) * chunks.size());
+      if (he == hipSuccess)
+        he = hipMemcpy(g.d_chunks, chunks.data(), sizeof(uint32_t) * chunks.size(),
+                       hipMemcpyHostToDevice);
+    }
+    if (he == hipSuccess) he = hipMalloc(&g.d_ent, sizeof(BatchEntry) * g.nent);
     if (he == hipSuccess) he = hipMalloc(&g.d_begins, sizeof(uint32_t) * g.nent);
     if (he == hipSuccess) he = hipMalloc(&g.d_ent_of, sizeof(uint32_t) * ent_of.size());
     if (he == hipSuccess)
@@ -511,9 +561,12 @@ hipError_t batch_enqueue(Batch &b, hipStream_t st) {
   for (size_t i = 0; i < b.groups.size() && e == hipSuccess; ++i) {
     const BatchGroup &g = b.groups[i];
     hipStream_t ls = b.lane[i] == 0 ? st : b.side[b.lane[i] - 1];
+    BatchQueue q;
+    q.counter = g.d_queue;
+    q.chunks = g.d_chunks;
+    q.nchunks = g.nchunks;
     e = launch_batch_group(g.dtype, g.variant, g.d_ent, g.d_begins, g.d_ent_of, g.nent,
-                           g.total_tiles,
-                           g.lds_bytes, g.grid, ls);
+                           g.total_tiles, g.lds_bytes, g.grid, ls, g.d_queue ? &q : nullptr);
   }
   for (size_t i = 0; i < b.side.size() && e == hipSuccess; ++i) {
     e = hipEventRecord(b.events[1 + i], b.side[i]);
@@ -550,7 +603,11 @@ void batch_free(Batch &b) {
     if (g.d_ent) (void)hipFree(g.d_ent);
     if (g.d_begins) (void)hipFree(g.d_begins);
     if (g.d_ent_of) (void)hipFree(g.d_ent_of);
+    if (g.d_queue) (void)hipFree(g.d_queue);
+    if (g.d_chunks) (void)hipFree(g.d_chunks);
     g.d_ent_of = nullptr;
+    g.d_queue = nullptr;
+    g.d_chunks = nullptr;
     g.d_ent = nullptr;
     g.d_begins = nullptr;
   }

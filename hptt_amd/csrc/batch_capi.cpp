@@ -70,7 +70,16 @@ extern "C" {
 
 int hpttx_batch_create(uint32_t count, const hpttx_batch_item *items, double tuning_timeout,
                        void *stream, hpttx_batch **out) {
+  return hpttx_batch_create_ex(count, items, tuning_timeout, stream, 0, out);
+}
+
+int hpttx_batch_create_ex(uint32_t count, const hpttx_batch_item *items, double tuning_timeout,
+                          void *stream, uint32_t flags, hpttx_batch **out) {
   if (!out) { hpttx::set_last_error("batch out-pointer is null"); return HPTTX_EINVAL; }
+  if (flags & ~(uint32_t)HPTTX_BATCH_ONE_LAUNCH) {
+    hpttx::set_last_error("unknown batch flags");
+    return HPTTX_EINVAL;
+  }
   *out = nullptr;
   if (count && !items) { hpttx::set_last_error("null item array"); return HPTTX_EINVAL; }
   std::vector<BatchItemSpec> specs(count);
@@ -81,7 +90,8 @@ int hpttx_batch_create(uint32_t count, const hpttx_batch_item *items, double tun
   hpttx_batch *h = new (std::nothrow) hpttx_batch();
   if (!h) { hpttx::set_last_error("out of host memory"); return HPTTX_ENOMEM; }
   std::string err;
-  int rc = batch_build(specs, static_cast<hipStream_t>(stream), tuning_timeout, h->b, err);
+  int rc = batch_build(specs, static_cast<hipStream_t>(stream), tuning_timeout,
+                       (flags & HPTTX_BATCH_ONE_LAUNCH) ? kBatchOneLaunch : 0u, h->b, err);
   if (rc != kOk) {
     delete h;
     hpttx::set_last_error(err);

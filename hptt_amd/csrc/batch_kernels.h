@@ -115,24 +115,85 @@ __device__ __forceinline__ void dispatch(uint32_t v, const BatchEntry &e, uint32
   }
 }
 
-// Batch kernel: a work-queue walker over the launch's tile slots (block b
-// takes slots b, b + G, b + 2G, ... for a grid of G blocks; launched with one
-// block per slot by default, or as a persistent grid of resident blocks);
-// each slot is renumbered to a tile so that one XCD runs neighbouring tiles.  CODE >= 0: every entry has that
-// variant (one instance per variant); CODE == kMerged: each entry's variant
-// is dispatched at run time (one instance per element type).
+// Batch kernel: a work-queue walker over the launch's tiles.  CODE >= 0:
+// every entry has that variant (one instance per variant); CODE == kMerged:
+// each entry's variant is dispatched at run time (one instance per element
+// type).  Two walks:
+//  * static (counter == null): block b takes slots b, b + G, b + 2G, ...
+//    for a grid of G blocks (one block per slot by default, or a persistent
+//    grid of resident blocks); each slot is renumbered to a tile so that one
+//    XCD runs neighbouring tiles;
+//  * dynamic (counter != null, a persistent grid): each block takes the
+//    index of its next chunk of consecutive tiles off a device counter (one
+//    atomic increment per chunk), until the chunks run out.
 constexpr int kMerged = -1;
+
+// Entry lookup cache of one block: tiles [first, end) belong to entry ei.
+struct EntryCursor {
+  uint32_t ei = 0, first = 0, end = 0;   // empty: forces a lookup
+};
+
+__device__ __forceinline__ void find_entry(const BatchEntry *__restrict__ ents,
+                                           const uint32_t *__restrict__ begins,
+                                           const uint32_t *__restrict__ ent_of, uint32_t nent,
+                                           uint32_t t, EntryCursor &c) {
+  if (t >= c.first && t < c.end) return;
+  if (ent_of) {
+    // the entry of every tile, tabulated at batch creation
+    c.ei = __builtin_amdgcn_readfirstlane(ent_of[t]);
+  } else {
+    // the last entry whose first tile is <= t
+    uint32_t lo = 0, hi = nent;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (__builtin_amdgcn_readfirstlane(begins[mid]) <= t) lo = mid;
+      else hi = mid;
+    }
+    c.ei = lo;
+  }
+  c.first = __builtin_amdgcn_readfirstlane(ents[c.ei].tile_begin);
+  c.end = c.first + __builtin_amdgcn_readfirstlane(ents[c.ei].ntiles);
+}
+
+template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
+__device__ __forceinline__ void run_tile(const BatchEntry &e, uint32_t lt, uint32_t tid,
+                                         unsigned char *smem, bool more) {
+  if constexpr (CODE == kMerged)
+    dispatch<T, S, NC, VCAP, CCOK, 0>(__builtin_amdgcn_readfirstlane(e.variant), e, lt, tid, smem,
+                                      more);
+  else
+    run_code<T, S, NC, VCAP, CCOK, CODE, kUcapBatch>(e, lt, tid, smem, more);
+}
 
 template <typename T, typename S, int NC, int VCAP, bool CCOK, int CODE>
 __global__ __launch_bounds__(kBlock) void k_batch(const BatchEntry *__restrict__ ents,
                                                   const uint32_t *__restrict__ begins,
                                                   const uint32_t *__restrict__ ent_of,
-                                                  uint32_t nent, uint32_t total) {
+                                                  uint32_t nent, uint32_t total,
+                                                  uint32_t *__restrict__ counter,
+                                                  const uint32_t *__restrict__ chunks,
+                                                  uint32_t nchunks) {
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint32_t s_chunk;
   const uint32_t tid = threadIdx.x;
+  EntryCursor cur;
+  if (counter) {
+    for (;;) {
+      if (tid == 0) s_chunk = atomicAdd(counter, 1u);
+      // also the barrier before the LDS is reused by this chunk's first tile
+      __syncthreads();
+      const uint32_t j = __builtin_amdgcn_readfirstlane(s_chunk);
+      __syncthreads();   // every thread has read s_chunk before it is rewritten
+      if (j >= nchunks) return;
+      const uint32_t lo = __builtin_amdgcn_readfirstlane(chunks[j]);
+      const uint32_t hi = __builtin_amdgcn_readfirstlane(chunks[j + 1]);
+      for (uint32_t t = lo; t < hi; ++t) {
+        find_entry(ents, begins, ent_of, nent, t, cur);
+        run_tile<T, S, NC, VCAP, CCOK, CODE>(ents[cur.ei], t - cur.first, tid, smem, t + 1 < hi);
+      }
+    }
+  }
   const uint32_t full64 = total & ~63u;
-  uint32_t ei = 0;
-  uint32_t first = 0, end = 0;   // tiles of entry ei: [first, end); empty forces a lookup
   const uint32_t nblocks = gridDim.x * gridDim.y;
   for (uint32_t i = blockIdx.y * gridDim.x + blockIdx.x; i < total; i += nblocks) {
     // Blocks are dispatched round-robin over the 8 XCDs, so slots i with
@@ -142,30 +203,9 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchEntry *__restrict__
     // edges, and each XCD has its own L2.  (A bijection on [0, total); the
     // last partial group keeps its order.)
     const uint32_t t = i < full64 ? (i & ~63u) | ((i & 7u) << 3) | ((i >> 3) & 7u) : i;
-    if (t < first || t >= end) {
-      if (ent_of) {
-        // the entry of every tile, tabulated at batch creation
-        ei = __builtin_amdgcn_readfirstlane(ent_of[t]);
-      } else {
-        // the last entry whose first tile is <= t
-        uint32_t lo = 0, hi = nent;
-        while (hi - lo > 1) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (__builtin_amdgcn_readfirstlane(begins[mid]) <= t) lo = mid;
-          else hi = mid;
-        }
-        ei = lo;
-      }
-      first = __builtin_amdgcn_readfirstlane(ents[ei].tile_begin);
-      end = first + __builtin_amdgcn_readfirstlane(ents[ei].ntiles);
-    }
-    const BatchEntry &e = ents[ei];
-    const bool more = i + nblocks < total;
-    if constexpr (CODE == kMerged)
-      dispatch<T, S, NC, VCAP, CCOK, 0>(__builtin_amdgcn_readfirstlane(e.variant), e, t - first,
-                                        tid, smem, more);
-    else
-      run_code<T, S, NC, VCAP, CCOK, CODE, kUcapBatch>(e, t - first, tid, smem, more);
+    find_entry(ents, begins, ent_of, nent, t, cur);
+    run_tile<T, S, NC, VCAP, CCOK, CODE>(ents[cur.ei], t - cur.first, tid, smem,
+                                         i + nblocks < total);
   }
 }
 

@@ -388,7 +388,8 @@ void Server::dispatch_loop() {
       const Group &g = groups[gi];
       he = launch_batch_group(g.dtype, g.variant, slot->d_ent + g.first, slot->d_begins + g.first,
                               nullptr, g.count, g.total, g.lds,
-                              batch_grid(g.dtype, g.variant, g.lds, g.total), slot->stream);
+                              batch_grid(g.dtype, g.variant, g.lds, g.total, batch_persistent()),
+                              slot->stream);
     }
     if (he == hipSuccess) he = hipEventRecord(slot->done, slot->stream);
     lk.lock();

@@ -184,6 +184,16 @@ typedef struct hpttx_batch_item {
 int hpttx_batch_create(uint32_t count, const hpttx_batch_item *items,
                        double tuning_timeout, void *stream,
                        hpttx_batch **batch);
+/* The same with a launch layout.  flags = 0: as hpttx_batch_create.
+ * HPTTX_BATCH_ONE_LAUNCH: one persistent launch per element type (the kernel
+ * dispatches each item's variant at run time), whose resident blocks take
+ * chunks of tiles, largest items first, off a device counter (a dynamic work
+ * queue, cleared by a kernel node before each launch); captured in the
+ * hipGraph like the default layout. */
+#define HPTTX_BATCH_ONE_LAUNCH 1u
+int hpttx_batch_create_ex(uint32_t count, const hpttx_batch_item *items,
+                          double tuning_timeout, void *stream, uint32_t flags,
+                          hpttx_batch **batch);
 /* Enqueue every item of the batch (graph replay) on the batch's stream. */
 int hpttx_batch_exec(hpttx_batch *batch);
 int hpttx_batch_exec_on(hpttx_batch *batch, void *stream);

@@ -227,3 +227,38 @@ def test_single_persistent_launch_per_type(device):
     batch.sync()
     for it, b in zip(items, bufs):
         assert oracle.first_mismatch(host(b["dB"]), _expected(it, b)) == -1
+
+
+def test_one_launch_dynamic_queue(device):
+    """The one-launch layout (hpttx_batch_create_ex with HPTTX_BATCH_ONE_LAUNCH):
+    one persistent launch per element type whose blocks take chunks of tiles
+    off a device counter.  The counter is cleared by a kernel node, so graph
+    replays (with eager work between them) and direct launches all match the
+    oracle."""
+    import torch
+    rng = np.random.default_rng(23)
+    items = _random_items(rng, 200, [0, 1, 2, 3], padded=True)
+    bufs = _materialize(items, 2100)
+    torch.cuda.synchronize()
+    bi = [hptt_amd.make_batch_item(b["dA"], b["dB"], it["size"], it["perm"], it["alpha"],
+                                   it["beta"], in_outer_size=it["io"] or (),
+                                   out_outer_size=it["oo"] or ())
+          for it, b in zip(items, bufs)]
+    batch = hptt_amd.Batch(bi, one_launch=True)
+    info = batch.info()
+    assert info.groups == len({it["code"] for it in items})
+    assert all(g < info.tiles for g in info.grid[:info.groups])
+    exps = [_expected(it, b) for it, b in zip(items, bufs)]
+    B0s = [b["dB"].clone() for b in bufs]
+    for rep in range(4):
+        for b, b0 in zip(bufs, B0s):
+            b["dB"].copy_(b0)
+        torch.cuda.synchronize()
+        batch.use_graph(rep != 3)
+        junk = torch.randn(1 << 20, device="cuda").sum()
+        batch.exec()
+        batch.sync()
+        torch.cuda.synchronize()
+        assert torch.isfinite(junk).item()
+        for k, (b, e) in enumerate(zip(bufs, exps)):
+            assert oracle.first_mismatch(host(b["dB"]), e) == -1, (rep, k)

@@ -2,11 +2,13 @@
 dims log-uniform in [2, 128], at most 4 Mi elements each, random perms,
 alpha/beta uniform in [-2, 2], beta = 0 for 25%), all fp32 and all fp64, in
 the default launch layout (one launch per element type, kernel variant and
-LDS class) and as merged persistent launches (one work-queue launch per
-element type).  Every tensor must equal its own plain plan->exec() bit for
-bit, nothing outside the tensors may be written, and at least one tensor of
-every (variant, LDS class) group -- the largest -- is checked against the
-oracle (the reference's RefTrans, test/inc/hptt/test_util.tcc:143-185)."""
+LDS class), as merged persistent launches (one static work-queue launch per
+element type) and in the one-launch layout (one persistent launch per
+element type over a dynamic queue).  Every tensor must equal its own plain
+plan->exec() bit for bit, nothing outside the tensors may be written, and at
+least one tensor of every (variant, LDS class) group -- the largest -- is
+checked against the oracle (the reference's RefTrans,
+test/inc/hptt/test_util.tcc:143-185)."""
 import os
 
 import numpy as np
@@ -18,17 +20,20 @@ from hptt_amd.workloads import batch_manifest, layout
 
 pytestmark = pytest.mark.gpu
 
+# layout: (environment while the batch is built, one_launch flag)
 LAYOUTS = {
-    "default": {},
-    "merged-persistent": {"HPTTX_BATCH_MODE": "merged", "HPTTX_BATCH_GRID": "persistent"},
+    "default": ({}, False),
+    "merged-persistent": ({"HPTTX_BATCH_MODE": "merged", "HPTTX_BATCH_GRID": "persistent"}, False),
+    "one-launch": ({}, True),
 }
 
 
-def _build(items, env):
+def _build(items, layout):
+    env, one_launch = layout
     old = {k: os.environ.get(k) for k in env}
     os.environ.update(env)
     try:
-        return hptt_amd.Batch(items)
+        return hptt_amd.Batch(items, one_launch=one_launch)
     finally:
         for k, v in old.items():
             if v is None:
@@ -65,7 +70,7 @@ def test_full_manifest(device, dtname, lay):
     torch.cuda.synchronize()
     batch = _build(items, LAYOUTS[lay])
     info = batch.info()
-    if lay == "merged-persistent":
+    if lay != "default":
         assert info.groups == 1
     batch.exec()
     batch.sync()
