@@ -359,6 +359,8 @@ def _bind_batch(L):
     L.hpttx_batch_item_path.argtypes = [vp, u32, ctypes.c_char_p, ctypes.c_size_t]
     L.hpttx_batch_destroy.restype = None
     L.hpttx_batch_destroy.argtypes = [vp]
+    L.hpttx_batch_run_noalloc.restype = ctypes.c_int
+    L.hpttx_batch_run_noalloc.argtypes = [u32, ctypes.POINTER(BatchItem), vp]
 
 
 def make_batch_item(in_data, out_data, in_size, perm, alpha, beta, dtype=None,
@@ -456,6 +458,15 @@ class Batch:
         if h is not None and h.value and _lib is not None:
             _lib.hpttx_batch_destroy(h)
             self._h = ctypes.c_void_p(0)
+
+
+def batch_run_noalloc(items, stream=None) -> None:
+    """Enqueue the transpositions of `items` (BatchItems) on `stream` without
+    allocating device memory: their tables go to the library's static device
+    scratch (hpttx_batch_run_noalloc in include/hptt_amd.h).  Calls whose
+    work overlaps on one GPU race on that scratch."""
+    arr = (BatchItem * len(items))(*items)
+    _check(lib().hpttx_batch_run_noalloc(len(items), arr, _stream_ptr(stream)))
 
 
 class ServerConfigC(ctypes.Structure):

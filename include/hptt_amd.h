@@ -230,6 +230,24 @@ typedef struct hpttx_batch_item_info {
 } hpttx_batch_item_info;
 int hpttx_batch_get_item_info(const hpttx_batch *batch, uint32_t index,
                               hpttx_batch_item_info *info);
+/*
+ * A batch run that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It plans
+ * `items` (as hpttx_batch_create with tuning_timeout 0), writes their tables
+ * into a static device array of HPTTX_NOALLOC_SCRATCH_BYTES (one per GPU,
+ * held until the process ends) and enqueues the one-launch layout on
+ * `stream` (one persistent launch per element type over a dynamic queue).
+ * Returns once the table upload has been read (it waits for `stream` to
+ * reach it); the transpositions themselves run asynchronously.  A batch
+ * whose tables do not fit the array is refused with HPTTX_EINVAL (about
+ * 9000 items fit).  Every call on one GPU shares that GPU's array: calls
+ * whose work overlaps on one GPU (another stream, or another thread before
+ * the previous call's launches finished) race on it.
+ */
+#define HPTTX_NOALLOC_SCRATCH_BYTES (8u << 20)
+int hpttx_batch_run_noalloc(uint32_t count, const hpttx_batch_item *items,
+                            void *stream);
 /* Path of item `index` ("tile", "tile-shared", "wide-copy", "axpby",
  * "gather") into buf. */
 int hpttx_batch_item_path(const hpttx_batch *batch, uint32_t index, char *buf,

@@ -201,3 +201,15 @@ def test_server_completion_bookkeeping_host():
                         "serve_book_test"], check=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and "SERVE BOOK OK" in r.stdout, r.stdout + r.stderr
+
+
+def test_noalloc_entry_point_source_calls_no_allocator():
+    """hpttx_batch_run_noalloc's source (and the batch launch helpers it
+    calls) names no device or pinned-host allocation routine."""
+    src = open(os.path.join(ROOT, "hptt_amd", "csrc", "noalloc.hip")).read()
+    code = re.sub(r"//[^\n]*", "", src)
+    for name in ("hipMalloc", "hipFree", "hipHostMalloc", "hipMallocAsync", "hipExtMalloc",
+                 "hipMallocManaged", "hipEventCreate", "hipStreamCreate"):
+        assert name not in code, name
+    assert "static __device__ __align__(256) unsigned char scratch[" in code
+    assert "hipGetSymbolAddress" in code
