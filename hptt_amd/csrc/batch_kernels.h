@@ -193,16 +193,12 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchEntry *__restrict__
       }
     }
   }
-  const uint32_t full64 = total & ~63u;
   const uint32_t nblocks = gridDim.x * gridDim.y;
   for (uint32_t i = blockIdx.y * gridDim.x + blockIdx.x; i < total; i += nblocks) {
-    // Blocks are dispatched round-robin over the 8 XCDs, so slots i with
-    // i % 8 == x run on XCD x (for a grid that is a multiple of 8 or covers
-    // every slot).  Renumber each group of 64 so that those are 8
-    // consecutive tiles: neighbouring tiles share cache lines at their
-    // edges, and each XCD has its own L2.  (A bijection on [0, total); the
-    // last partial group keeps its order.)
-    const uint32_t t = i < full64 ? (i & ~63u) | ((i & 7u) << 3) | ((i >> 3) & 7u) : i;
+    // XCD-aware order (for a grid that is a multiple of 8 or covers every
+    // slot): neighbouring tiles share cache lines at their edges, and each
+    // XCD has its own L2.
+    const uint32_t t = xcd_tile(i, total);
     find_entry(ents, begins, ent_of, nent, t, cur);
     run_tile<T, S, NC, VCAP, CCOK, CODE>(ents[cur.ei], t - cur.first, tid, smem,
                                          i + nblocks < total);

@@ -137,16 +137,28 @@ struct TileDesc {
   uint32_t nvr;          // vectors per read run  (ceil(TR*VS / vr))
   uint32_t nvw;          // vectors per write run (ceil(TC*VS / vw))
   uint32_t vr, vw;       // vector widths in elements
-  // Run mode per side: 0 = exact (every run starts 16-byte aligned and is a
-  // whole number of vectors), 1 = unaligned (vectors at any element offset,
-  // a run's last partial vector done element by element).
-  uint32_t ua_r, ua_w;
+  // Mode bits, packed into one word (the descriptor is passed by value as a
+  // kernel argument; small plans are launch-bound, and 16 more bytes of it
+  // measured 0.3 us more per launch on a loop of 4096 small plans).
+  //  ua_r / ua_w: run mode per side: 0 = exact (every run starts 16-byte
+  //    aligned and is a whole number of vectors), 1 = unaligned (vectors at
+  //    any element offset, a run's last partial vector element by element).
+  //  xcd: tile order; 1 = renumber tiles so that each XCD runs 8 consecutive
+  //    ones (set when neighbouring tiles share 128-byte lines, so that the
+  //    shared lines stay in one XCD's L2).
+  //  sh_vec: 1 = shared group of whole write vectors (VS % vw == 0, rows
+  //    unswizzled and vector-aligned): the write pass reads each output
+  //    vector from LDS whole.
+  uint32_t ua_r : 1, ua_w : 1, xcd : 1, sh_vec : 1;
   // LDS chunk swizzle (rows of a power-of-two number of 16-byte chunks, no
-  // padding): chunk k of LDS row `row` is stored at k ^ ((row >> swz_shift)
-  // & swz_mask).  The read pass's vector stores stay conflict-free and the
-  // write pass's column gathers (consecutive lanes on rows VW apart) spread
-  // over the banks; swz_mask = 0 is the plain padded layout.
-  uint32_t swz_mask, swz_shift;
+  // padding): chunk k of LDS row `row` is stored at k ^ ((row >> log2(W)) &
+  // swz_mask), W = 16 / sizeof(element) elements per chunk.  The read pass's vector stores stay conflict-free and the
+  // write pass's column reads (consecutive lanes on rows VW apart) spread
+  // over the banks; swz_mask = 0 is the plain padded layout.  (A rotation
+  // for other row lengths, (k + s) mod n, cut the batched fp32 workload's
+  // LDS bank conflicts from 4.5 to 1.9 cycles per LDS instruction but not
+  // its time, and its per-access select made small plans 5% slower.)
+  uint32_t swz_mask;
   FastDiv div_nR, div_nC, div_nvr, div_TC, div_nvw, div_TR, div_VS;
   FastDiv div_ngw;       // groups of 16 / esize R indices per tile
   DimList R, C, O;
@@ -316,6 +328,21 @@ HPTTX_HD TileCoord tile_coord(const TileDesc &d, uint32_t t) {
   return tc;
 }
 
+// Position of 16-byte chunk k of LDS row `row` (see TileDesc::swz_mask).
+template <typename T>
+HPTTX_HD uint32_t lds_chunk(const TileDesc &d, uint32_t row, uint32_t k) {
+  constexpr uint32_t kShift = sizeof(T) >= 16 ? 0 : sizeof(T) == 8 ? 1 : 2;
+  return k ^ ((row >> kShift) & d.swz_mask);
+}
+
+// XCD-aware tile order: blocks are dispatched round-robin over the 8 XCDs,
+// so slots i with i % 8 == x run on XCD x.  Renumbering each group of 64
+// slots makes those 8 consecutive tiles (a bijection on [0, n); the last
+// partial group keeps its order).
+HPTTX_HD uint32_t xcd_tile(uint32_t i, uint32_t n) {
+  return i < (n & ~63u) ? (i & ~63u) | ((i & 7u) << 3) | ((i >> 3) & 7u) : i;
+}
+
 // LDS footprint of a tile, in bytes: data rows then the four offset tables.
 inline size_t tile_lds_bytes(const TileDesc &d, size_t esize) {
   size_t data = (size_t)d.TO * d.TC * d.lds_row * esize;
@@ -424,7 +451,7 @@ HPTTX_HD void tile_read(const TileDesc &d, const TileCoord &tc, uint32_t tid,
       const uint32_t o = fdiv(row, d.div_TC);
       const uint32_t c = row - o * d.TC;
       ok[u] = (k < items) && (c < tc.clen) && (v * VR < run);
-      lidx[u] = row * d.lds_row + (v ^ ((row >> d.swz_shift) & d.swz_mask)) * VR;
+      lidx[u] = row * d.lds_row + lds_chunk<T>(d, row, v) * VR;
       if (ok[u]) {
         const HPTTX_GLOBAL T *src = A + abase + v * VR + tb.AC[c] + tb.AO[o];
         if (!UA || v * VR + VR <= run)
@@ -514,25 +541,32 @@ HPTTX_HD void tile_write_finish(const TileDesc &d, const TileCoord &tc,
     const WriteAddr a = write_addr(d, tc, w.k0 + (uint32_t)u * NTH, VW, tb);
     if (!a.ok) continue;
     T x[VW];
+    if (SH && d.sh_vec) {
+      // A shared group of whole vectors (VS % VW == 0, unswizzled rows):
+      // the vector's VW elements are consecutive s of one LDS row.
+      const uint32_t c = fdiv(a.q0, d.div_VS);
+      const uint32_t s = a.q0 - c * d.VS;
+      vload<T, VW>(lds + (a.lrow0 + c) * d.lds_row + a.x0 + s, x);
+    } else {
 #pragma unroll
-    for (int j = 0; j < VW; ++j) {
-      // elements past a partial vector's end read element 0 of the run
-      // (in the tile, never stored)
-      const uint32_t q = (!UA || (uint32_t)j < a.n) ? a.q0 + j : 0;
-      uint32_t c = q, s = 0;
-      if (SH) {
-        c = fdiv(q, d.div_VS);
-        s = q - c * d.VS;
+      for (int j = 0; j < VW; ++j) {
+        // elements past a partial vector's end read element 0 of the run
+        // (in the tile, never stored)
+        const uint32_t q = (!UA || (uint32_t)j < a.n) ? a.q0 + j : 0;
+        uint32_t c = q, s = 0;
+        if (SH) {
+          c = fdiv(q, d.div_VS);
+          s = q - c * d.VS;
+        }
+        // LDS element (row lrow0 + c, x0 + s), through the chunk swizzle
+        // (chunks of 16 / sizeof(T) elements, the read pass's vectors)
+        constexpr uint32_t kChunk = sizeof(T) >= 16 ? 1 : 16 / sizeof(T);
+        const uint32_t lrow = a.lrow0 + c, xc = a.x0 + s;
+        const uint32_t xs = lds_chunk<T>(d, lrow, xc / kChunk) * kChunk + xc % kChunk;
+        const T *src = lds + lrow * d.lds_row + xs;
+        HPTTX_ACCESS(src, sizeof(T), false);
+        x[j] = *src;
       }
-      // LDS element (row lrow0 + c, x0 + s), through the chunk swizzle
-      // (chunks of 16 / sizeof(T) elements, the read pass's vectors)
-      constexpr uint32_t kChunk = sizeof(T) >= 16 ? 1 : 16 / sizeof(T);
-      const uint32_t lrow = a.lrow0 + c, xc = a.x0 + s;
-      const uint32_t xs = ((xc / kChunk) ^ ((lrow >> d.swz_shift) & d.swz_mask)) * kChunk +
-                          xc % kChunk;
-      const T *src = lds + lrow * d.lds_row + xs;
-      HPTTX_ACCESS(src, sizeof(T), false);
-      x[j] = *src;
     }
     T out[VW];
 #pragma unroll
@@ -602,7 +636,7 @@ HPTTX_HD void tile_write_quad(const TileDesc &d, const TileCoord &tc, uint32_t t
 #pragma unroll
     for (uint32_t m = 0; m < W; ++m) {
       const uint32_t row = o * d.TC + c0 + m;
-      const uint32_t chunk = g ^ ((row >> d.swz_shift) & d.swz_mask);
+      const uint32_t chunk = lds_chunk<T>(d, row, g);
       vload<T, W>(lds + row * d.lds_row + chunk * W, x[m]);
     }
 #pragma unroll

@@ -31,8 +31,9 @@ template <typename T, typename S, int NC, int VR, int VW, bool UR, bool UW, bool
 __global__ __launch_bounds__(NTH) void k_tile(TileDesc d, const T *__restrict__ A,
                                               T *__restrict__ B, Coef<S> co) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const uint32_t t = blockIdx.y * gridDim.x + blockIdx.x;
-  if (t >= d.ntiles) return;
+  const uint32_t i = blockIdx.y * gridDim.x + blockIdx.x;
+  if (i >= d.ntiles) return;
+  const uint32_t t = d.xcd ? xcd_tile(i, d.ntiles) : i;
   T *lds = reinterpret_cast<T *>(smem);
   const TileTabs tb = tile_tabs(d, smem, sizeof(T));
   const uint32_t tid = threadIdx.x;

@@ -52,6 +52,11 @@ struct Problem {
 
 enum Kind : int { kTile = 0, kRows = 1 };
 
+// MI355X's last-level cache (Infinity Cache): problems whose A and B exceed
+// it stream from HBM, and the planner's cache-related choices (streaming
+// mode, line-aligned runs, XCD-aware tile order) apply to them only.
+constexpr uint64_t kLastLevelCacheBytes = 256ull << 20;
+
 // Tiles whose LDS footprint lets at most 4 blocks share a CU (more than
 // 32 KiB of its 160 KiB) run 512-thread blocks, so those 4 blocks still fill
 // all 32 wave slots; smaller tiles keep kBlock threads and more blocks.
