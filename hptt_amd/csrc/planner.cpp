@@ -153,7 +153,7 @@ TuneParams default_tune(const Problem &p) {
   t.run_r = 256 / p.esize;
   t.run_c = (p.esize == 8 ? 1024 : 512) / p.esize;
   t.block_elems = 8192 / p.esize;
-  t.wide_min_bytes = 256;
+  t.wide_min_bytes = 128;
   return t;
 }
 
@@ -497,8 +497,9 @@ int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
     rc = plan_rows(p, p.border[0], tp, lp, err);   // no unit-stride dim (padded size-1 dims)
   } else if (p.border[0] == 0) {
     // Stride-1 dim kept (reference "common leading",
-    // parameter_trans.tcc:154-158).  Long rows: wide copy; short rows: TILE
-    // with the rows as a shared innermost group.
+    // parameter_trans.tcc:154-158).  Rows of at least wide_min_bytes (128 by
+    // default): wide copy; shorter rows: TILE with the rows as a shared
+    // innermost group.
     const uint64_t L = p.d[0].n;
     const bool contiguous_after = p.d[1].sa == (int64_t)L && p.d[p.border[1]].sb == (int64_t)L;
     if (L * p.esize >= tp.wide_min_bytes || !contiguous_after || p.border[1] == 1)
@@ -511,9 +512,10 @@ int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
   if (rc != kOk) return rc;
   // Streaming mode.  Model: on when the problem cannot stay in the 256 MiB
   // last-level cache anyway and every access belongs to a run of at least
-  // 128 bytes (tile runs) or 256 bytes (rows); exhaustive tuning of the
-  // large 6-D and 4-D workloads chose it for every winning tile shape.  The
-  // tuner may override.
+  // 128 bytes; exhaustive tuning of the large 6-D and 4-D workloads chose it
+  // for every winning tile shape, and of the reference's shapes with kept
+  // rows of 128-255 bytes the wide copy with it (192-byte rows: 2.9 ->
+  // 5.3 TB/s against a shared-group tile).  The tuner may override.
   if (tp.stream == 1) {
     lp.stream = true;
   } else if (tp.stream == 2) {
@@ -525,7 +527,7 @@ int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
       long_runs = (uint64_t)lp.tile.TR * lp.tile.VS * p.esize >= 128 &&
                   (uint64_t)lp.tile.TC * lp.tile.VS * p.esize >= 128;
     else
-      long_runs = lp.rows.ra == 1 && lp.rows.rb == 1 && (uint64_t)lp.rows.L * p.esize >= 256;
+      long_runs = lp.rows.ra == 1 && lp.rows.rb == 1 && (uint64_t)lp.rows.L * p.esize >= 128;
     lp.stream = long_runs && bytes > kLastLevelCacheBytes;
   }
   finalize_vectors(p, lp, lp.vr_shape, lp.vw_shape);
