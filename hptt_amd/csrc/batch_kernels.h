@@ -38,7 +38,7 @@ __device__ __forceinline__ void tile_one(const TileDesc &d, uint32_t lt, uint32_
   const TileCoord tc = tile_coord(d, lt);
   tile_tables(d, tc, tid, tb);
   __syncthreads();
-  tile_read<T, VR, NT, UCAP, UR>(d, tc, tid, A, lds, tb);
+  tile_read<T, VR, NT, unroll_for<T, VR, UCAP>(), UR>(d, tc, tid, A, lds, tb);
   __syncthreads();
   if constexpr (quad_write<T, UW, SH>())
     tile_write_quad<T, S, NC, UPD, CC, NT, kBlock>(d, tc, tid, B, lds, tb, co);

@@ -40,7 +40,7 @@ __global__ __launch_bounds__(NTH) void k_tile(TileDesc d, const T *__restrict__ 
   const TileCoord tc = tile_coord(d, t);
   tile_tables<NTH>(d, tc, tid, tb);
   __syncthreads();
-  tile_read<T, VR, NT, kUcapPlan, UR, NTH>(d, tc, tid, (const HPTTX_GLOBAL T *)A, lds, tb);
+  tile_read<T, VR, NT, kReadGroupPlan, UR, NTH>(d, tc, tid, (const HPTTX_GLOBAL T *)A, lds, tb);
   __syncthreads();
   if constexpr (quad_write<T, UW, SH>())
     tile_write_quad<T, S, NC, UPD, CC, NT, NTH>(d, tc, tid, (HPTTX_GLOBAL T *)B, lds, tb, co);

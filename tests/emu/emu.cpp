@@ -78,8 +78,12 @@ void run_tile(const LaunchPlan &lp, const T *A, T *B, const Coef<S> &k) {
     // fill shows up in B
     std::memset(smem.data(), 0xff, lp.lds_bytes);
     for (uint32_t tid = 0; tid < NTH; ++tid) tile_tables<NTH>(d, tc, tid, tb);
+    // the single-plan read pass, then the batch kernels' (groups of
+    // unroll_for loads): both fill the same LDS slots, and both are checked
     for (uint32_t tid = 0; tid < NTH; ++tid)
-      tile_read<T, VR, false, kUcapPlan, UR, NTH>(d, tc, tid, A, lds, tb);
+      tile_read<T, VR, false, kReadGroupPlan, UR, NTH>(d, tc, tid, A, lds, tb);
+    for (uint32_t tid = 0; tid < NTH; ++tid)
+      tile_read<T, VR, false, unroll_for<T, VR, kUcapBatch>(), UR, NTH>(d, tc, tid, A, lds, tb);
     for (uint32_t tid = 0; tid < NTH; ++tid) {
       if constexpr (quad_write<T, UW, SH>())
         tile_write_quad<T, S, NC, UPD, CC, false, NTH>(d, tc, tid, B, lds, tb, k);
