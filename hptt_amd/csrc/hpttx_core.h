@@ -146,8 +146,9 @@ struct TileDesc {
   //    aligned and is a whole number of vectors), 1 = unaligned (vectors at
   //    any element offset, a run's last partial vector element by element).
   //  xcd: tile order; 1 = renumber tiles so that each XCD runs 8 consecutive
-  //    ones (set when neighbouring tiles share 128-byte lines, so that the
-  //    shared lines stay in one XCD's L2).
+  //    ones (set for problems beyond the last-level cache: neighbouring
+  //    tiles share rows, lines and DRAM pages, which one XCD then fetches
+  //    together).
   //  sh_vec: 1 = shared group of whole write vectors (VS % vw == 0, rows
   //    unswizzled and vector-aligned): the write pass reads each output
   //    vector from LDS whole.
