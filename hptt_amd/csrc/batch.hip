@@ -52,7 +52,8 @@ uint32_t entry_variant(const LaunchPlan &lp, bool update, bool cc, uint32_t vcap
   }
   if (update) v |= kVarUpd;
   if (cc) v |= kVarCc;
-  if (lp.stream) v |= kVarNt;
+  // the batch kernels stream loads and stores alike, or neither
+  if (lp.nt_load && lp.nt_store) v |= kVarNt;
   return v;
 }
 
@@ -69,6 +70,7 @@ void pick_vectors(const Problem &p, LaunchPlan &lp, const void *A, const void *B
     if (vw > align_vec(B, p.esize, cap) || vw != cap) vw = 1;
   }
   finalize_vectors(p, lp, vr, vw);
+  stream_for_pointers(p, lp, A, B);
 }
 
 }  // namespace

@@ -58,6 +58,7 @@ void refresh_vectors(hpttx_plan *p) {
     if (vw > align_vec(p->B, esz, cap) || vw != cap) vw = 1;
     p->cur = p->shape;
     finalize_vectors(p->prob, p->cur, vr, vw);
+    stream_for_pointers(p->prob, p->cur, p->A, p->B);
   }
 }
 

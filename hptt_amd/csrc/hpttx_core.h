@@ -611,7 +611,7 @@ HPTTX_HD WriteAddr write_addr(const TileDesc &d, const TileCoord &tc, uint32_t k
 }
 
 template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT, int UCAP,
-          bool UA, uint32_t NTH, bool WHOLE>
+          bool UA, uint32_t NTH, bool WHOLE, bool NTS>
 HPTTX_HD void tile_write_pass(const TileDesc &d, const TileCoord &tc, uint32_t tid,
                               HPTTX_GLOBAL T *HPTTX_RESTRICT B, const T *lds,
                               const TileTabs &tb, const Coef<S> &k) {
@@ -667,24 +667,27 @@ HPTTX_HD void tile_write_pass(const TileDesc &d, const TileCoord &tc, uint32_t t
         __builtin_memcpy(&out[j], &re, sizeof(T));
       }
       if (a[u].n == (uint32_t)VW)
-        gstore<T, VW, NT, UA>(B + a[u].boff, out);
+        gstore<T, VW, NTS, UA>(B + a[u].boff, out);
       else if (UA && a[u].n)
-        store_part<T, VW, NT>(B + a[u].boff, a[u].n, out);
+        store_part<T, VW, NTS>(B + a[u].boff, a[u].n, out);
     }
   };
   for_groups<U, NTH>(tid, items, group);
 }
 
-// The whole write pass.
+// The whole write pass.  NT: B's loads carry the non-temporal hint; NTS: its
+// stores do.
 template <typename T, typename S, int NC, int VW, bool UPD, bool CC, bool SH, bool NT, int UCAP,
-          bool UA = false, uint32_t NTH = kBlock>
+          bool UA = false, uint32_t NTH = kBlock, bool NTS = NT>
 HPTTX_HD void tile_write(const TileDesc &d, const TileCoord &tc, uint32_t tid,
                          HPTTX_GLOBAL T *HPTTX_RESTRICT B, const T *lds, const TileTabs &tb,
                          const Coef<S> &k) {
   if (!UA || tc.clen * d.VS >= (uint32_t)VW)
-    tile_write_pass<T, S, NC, VW, UPD, CC, SH, NT, UCAP, UA, NTH, true>(d, tc, tid, B, lds, tb, k);
+    tile_write_pass<T, S, NC, VW, UPD, CC, SH, NT, UCAP, UA, NTH, true, NTS>(d, tc, tid, B, lds,
+                                                                              tb, k);
   else
-    tile_write_pass<T, S, NC, VW, UPD, CC, SH, NT, UCAP, UA, NTH, false>(d, tc, tid, B, lds, tb, k);
+    tile_write_pass<T, S, NC, VW, UPD, CC, SH, NT, UCAP, UA, NTH, false, NTS>(d, tc, tid, B, lds,
+                                                                               tb, k);
 }
 
 // Write pass by W x W blocks (W = 16 / sizeof(T): 4 for 4-byte and 2 for
@@ -697,7 +700,7 @@ HPTTX_HD void tile_write(const TileDesc &d, const TileCoord &tc, uint32_t tid,
 // pass this is 1/W of the LDS instructions and of the per-element address
 // arithmetic.  Exact writes make every C run a whole number of vectors
 // (clen % W == 0); R indices past rlen (a partial last group) are skipped.
-template <typename T, typename S, int NC, bool UPD, bool CC, bool NT, uint32_t NTH>
+template <typename T, typename S, int NC, bool UPD, bool CC, bool NT, uint32_t NTH, bool NTS = NT>
 HPTTX_HD void tile_write_quad(const TileDesc &d, const TileCoord &tc, uint32_t tid,
                               HPTTX_GLOBAL T *HPTTX_RESTRICT B, const T *lds,
                               const TileTabs &tb, const Coef<S> &k) {
@@ -739,7 +742,7 @@ HPTTX_HD void tile_write_quad(const TileDesc &d, const TileCoord &tc, uint32_t t
         Elem<S, NC> re = axpby<S, NC, UPD, CC>(xe, ye, k);
         __builtin_memcpy(&out[m], &re, sizeof(T));
       }
-      gstore<T, W, NT>(B + boff[j], out);
+      gstore<T, W, NTS>(B + boff[j], out);
     }
   }
 }

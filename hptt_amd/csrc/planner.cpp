@@ -484,6 +484,32 @@ void finalize_vectors(const Problem &p, LaunchPlan &lp, uint32_t vr, uint32_t vw
   (void)vw;
 }
 
+bool tile_whole_lines(const Problem &p, const TileDesc &t) {
+  auto lines = [&](uint64_t elems) { return (elems * p.esize) % 128 == 0; };
+  if (!lines((uint64_t)t.TR * t.VS) || !lines((uint64_t)t.VR * t.VS) ||
+      !lines((uint64_t)t.TC * t.VS) || !lines((uint64_t)t.VC * t.VS))
+    return false;
+  for (uint32_t k = 0; k < t.C.nd; ++k)
+    if (!lines((uint64_t)t.C.sa[k])) return false;
+  for (uint32_t k = 0; k < t.R.nd; ++k)
+    if (!lines((uint64_t)t.R.sb[k])) return false;
+  for (uint32_t k = 0; k < t.O.nd; ++k)
+    if (!lines((uint64_t)t.O.sa[k]) || !lines((uint64_t)t.O.sb[k])) return false;
+  return true;
+}
+
+// Non-temporal choice of a tile plan whose runs are not all whole lines.
+static void stream_partial_lines(const Problem &p, LaunchPlan &lp) {
+  lp.nt_load = p.esize == 4;
+  lp.nt_store = false;
+}
+
+void stream_for_pointers(const Problem &p, LaunchPlan &lp, const void *A, const void *B) {
+  if (lp.kind == kTile && lp.nt_load && lp.nt_store && lp.tune.stream == 0 &&
+      (((uintptr_t)A | (uintptr_t)B) % 128) != 0)
+    stream_partial_lines(p, lp);
+}
+
 int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
                 std::string &err, uint32_t min_tiles) {
   TuneParams tp = tp_in;
@@ -516,20 +542,31 @@ int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
   // 128 bytes; exhaustive tuning of the large 6-D and 4-D workloads chose it
   // for every winning tile shape, and of the reference's shapes with kept
   // rows of 128-255 bytes the wide copy with it (192-byte rows: 2.9 ->
-  // 5.3 TB/s against a shared-group tile).  The tuner may override.
-  if (tp.stream == 1) {
-    lp.stream = true;
+  // 5.3 TB/s against a shared-group tile).  Tiles whose runs are not all
+  // whole 128-byte lines stream loads only (4-byte elements) or nothing
+  // (stream_partial_lines; the pointers' part is checked when the plan gets
+  // its data, stream_for_pointers).  The tuner may override.
+  const bool rows = lp.kind == kRows;
+  if (tp.stream == 1 || (rows && tp.stream >= 3)) {
+    lp.nt_load = lp.nt_store = true;
   } else if (tp.stream == 2) {
-    lp.stream = false;
+    lp.nt_load = lp.nt_store = false;
+  } else if (tp.stream == 3) {
+    lp.nt_load = true;
+    lp.nt_store = false;
+  } else if (tp.stream == 4) {
+    lp.nt_load = false;
+    lp.nt_store = true;
   } else {
     const uint64_t bytes = p.N * p.esize * 2;
     bool long_runs;
-    if (lp.kind == kTile)
+    if (!rows)
       long_runs = (uint64_t)lp.tile.TR * lp.tile.VS * p.esize >= 128 &&
                   (uint64_t)lp.tile.TC * lp.tile.VS * p.esize >= 128;
     else
       long_runs = lp.rows.ra == 1 && lp.rows.rb == 1 && (uint64_t)lp.rows.L * p.esize >= 128;
-    lp.stream = long_runs && bytes > kLastLevelCacheBytes;
+    lp.nt_load = lp.nt_store = long_runs && bytes > kLastLevelCacheBytes;
+    if (lp.nt_load && !rows && !tile_whole_lines(p, lp.tile)) stream_partial_lines(p, lp);
   }
   finalize_vectors(p, lp, lp.vr_shape, lp.vw_shape);
   if (lp.lds_bytes > 65536) { err = "tile needs more than 64 KiB of LDS"; return kEUnsupported; }
@@ -559,7 +596,8 @@ std::string describe(const Problem &p, const LaunchPlan &lp) {
        << " seg=" << r.SEG << " rows/block=" << r.RB << " vec=" << r.v << (r.ua ? "u" : "")
        << " tiles=" << r.ntiles;
   }
-  os << "\ngrid: " << lp.grid << " x " << lp.threads << "  streaming: " << (lp.stream ? "on" : "off")
+  os << "\ngrid: " << lp.grid << " x " << lp.threads << "  streaming: "
+     << (lp.nt_load ? (lp.nt_store ? "on" : "loads") : (lp.nt_store ? "stores" : "off"))
      << "  tune(run_r,run_c,block,wide,stream)=" << lp.tune.run_r << "," << lp.tune.run_c << ","
      << lp.tune.block_elems << "," << lp.tune.wide_min_bytes << "," << lp.tune.stream << "\n";
   return os.str();

@@ -21,7 +21,9 @@ struct TuneParams {
   uint32_t run_c = 0;        // target write-run length along B, in elements
   uint32_t block_elems = 0;  // target elements per block tile
   uint32_t wide_min_bytes = 0;  // stride-1-kept rows at least this long use ROWS
-  uint32_t stream = 0;       // non-temporal loads/stores: 0 = model, 1 = on, 2 = off
+  uint32_t stream = 0;       // non-temporal accesses: 0 = model, 1 = loads and stores,
+                             // 2 = none, 3 = loads only, 4 = stores only (tiles; rows
+                             // take 3 and 4 as 1)
   bool operator==(const TuneParams &o) const {
     return run_r == o.run_r && run_c == o.run_c && block_elems == o.block_elems &&
            wide_min_bytes == o.wide_min_bytes && stream == o.stream;
@@ -71,7 +73,8 @@ struct LaunchPlan {
   uint32_t grid = 0;
   size_t lds_bytes = 0;
   uint32_t threads = kBlock;   // threads per block (kBlock or kWideBlock)
-  bool stream = false;         // non-temporal loads and stores
+  bool nt_load = false;        // non-temporal loads (A's, and B's when updating)
+  bool nt_store = false;       // non-temporal stores
   TuneParams tune{};
   std::string path;            // "tile", "tile-shared", "wide-copy", "axpby", "gather"
 };
@@ -107,6 +110,23 @@ int plan_launch(const Problem &p, const TuneParams &tp, LaunchPlan &lp,
 // width 16 / esize select exact vectors on that side, anything less
 // unaligned full-width vectors (scalars for 16-byte elements).
 void finalize_vectors(const Problem &p, LaunchPlan &lp, uint32_t vr, uint32_t vw);
+
+// Streaming (non-temporal) accesses of a tile plan: loads and stores where
+// every run starts and ends on a 128-byte line; where some do not (a run
+// that ends inside a line leaves the rest of the line to a neighbouring
+// tile), loads only for 4-byte elements and none for wider ones.  Measured
+// on MI355X (problems beyond the last-level cache, against loads and stores
+// streamed): the fp32 {3,2,1,0} reversal of {97,97,97,97} 3.41 -> 4.32 TB/s
+// with loads only, the reference's {48,352,4,28,28} perm {1,3,0,4,2} 5.87 ->
+// 6.21 and {2320,384,59} perm {1,0,2} 6.19 -> 6.24; the c128 reversals of
+// {97,97,97,97} 3.68 -> 4.16 and {90,96,96,110} 4.02 -> 4.51 with none (loads
+// only: 3.41 and 3.66); on runs of whole lines both stay 5% ahead of either
+// alone ({96,96,96,96} and {64,128,128,64} c128).
+// tile_whole_lines: the shape's part (run lengths and the strides that place
+// run starts); stream_for_pointers: the same choice for pointers that are
+// not 128-byte aligned (a tuned choice is kept).
+bool tile_whole_lines(const Problem &p, const TileDesc &t);
+void stream_for_pointers(const Problem &p, LaunchPlan &lp, const void *A, const void *B);
 
 // Widest vector (elements) a pointer's alignment allows for this element size.
 uint32_t align_vec(const void *ptr, uint32_t esize, uint32_t cap);

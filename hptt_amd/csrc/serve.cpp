@@ -190,14 +190,16 @@ void Server::stop() {
 
 int Server::plan_cached(const BatchItemSpec &spec, PlannedItem &q, std::string &err) {
   // Plans depend on shape, perm, padding, update / complex-coefficient flags
-  // and the pointers' 16-byte alignment, not on the coefficient values.
+  // and the pointers' 16- and 128-byte alignment (vectors; streaming), not
+  // on the coefficient values.
   const bool cc = spec.cc && (spec.coef.ai != 0.0 || spec.coef.bi != 0.0);
   const bool upd = cc ? (std::fabs(spec.coef.br) >= 1e-16 || std::fabs(spec.coef.bi) >= 1e-16)
                       : std::fabs(spec.coef.br) >= 1e-16;
   const uint32_t es = dtype_esize(spec.dtype);
   std::ostringstream os;
   os << spec.dtype << '|' << spec.rank << '|' << upd << '|' << cc << '|'
-     << (es ? align_vec(spec.A, es, 16 / es) : 0) << '|' << (es ? align_vec(spec.B, es, 16 / es) : 0);
+     << (es ? align_vec(spec.A, es, 16 / es) : 0) << '|' << (es ? align_vec(spec.B, es, 16 / es) : 0)
+     << '|' << ((((uintptr_t)spec.A | (uintptr_t)spec.B) % 128) == 0);
   for (uint32_t d = 0; d < spec.rank && d < (uint32_t)kMaxRank; ++d)
     os << '|' << spec.in_size[d] << ',' << spec.perm[d] << ',' << spec.in_outer[d] << ','
        << spec.out_outer[d];

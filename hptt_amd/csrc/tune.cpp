@@ -30,7 +30,7 @@ const char *cache_path() {
 bool plausible(const TuneParams &t) {
   const uint32_t kMax = 1u << 20;
   return t.run_r <= kMax && t.run_c <= kMax && t.block_elems <= kMax &&
-         t.wide_min_bytes <= kMax && t.stream <= 2;
+         t.wide_min_bytes <= kMax && t.stream <= 4;
 }
 
 void load_file_locked() {
@@ -83,7 +83,7 @@ bool tune_lookup(const Problem &p, bool update, bool cc, TuneParams &out) {
 
 bool same_launch(const LaunchPlan &a, const LaunchPlan &b) {
   if (a.kind != b.kind) return false;
-  if (a.stream != b.stream) return false;
+  if (a.nt_load != b.nt_load || a.nt_store != b.nt_store) return false;
   if (a.kind == kTile)
     return a.tile.TR == b.tile.TR && a.tile.TC == b.tile.TC && a.tile.TO == b.tile.TO &&
            a.tile.VR == b.tile.VR && a.tile.VC == b.tile.VC && a.tile.VS == b.tile.VS;
@@ -125,7 +125,7 @@ std::vector<TuneParams> tune_candidates(const Problem &p) {
     for (uint32_t rc : runs)
       for (uint32_t be : blocks)
         for (uint32_t w : wides)
-          for (uint32_t st : {1u, 2u}) {
+          for (uint32_t st : {1u, 2u, 3u}) {
             TuneParams t{std::max<uint32_t>(rr, 4), std::max<uint32_t>(rc, 4), be, w, st};
             if (!(t == d)) c.push_back(t);
           }
