@@ -243,6 +243,98 @@ def time_batched(dtype_name, reps=5, seq_reps=3, tune=0.0, one_launch=False):
     return res
 
 
+def time_serving_manifest(dtype_name, passes=3, seq_reps=3, max_batch=4096, slots=3):
+    """The served mode on the batched workload's mix: the 4096-tensor manifest
+    handed to the continuous-batching server in one hpttx_server_submit_many
+    call, closed loop, against one plain plan->exec() per tensor in sequence
+    (plans created beforehand, back to back on one stream).  Wall time is the
+    host clock from before the submission to the last completion; the first
+    pass plans every request (cold plan cache), later passes find the plans
+    cached.  Served results must equal the sequential ones bit for bit."""
+    import torch
+
+    import hptt_amd
+    from hptt_amd.workloads import batch_manifest, layout
+    man = batch_manifest()
+    tdt = getattr(torch, dtype_name)
+    esz = torch.empty(0, dtype=tdt).element_size()
+    offs, total = layout(man, esz)
+    g = torch.Generator(device="cuda").manual_seed(4096)
+    A = torch.empty(total, dtype=tdt, device="cuda").uniform_(-1, 1, generator=g)
+    B0 = torch.empty(total, dtype=tdt, device="cuda").uniform_(-1, 1, generator=g)
+    B = B0.clone()
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    items, plans = [], []
+    for m, o in zip(man, offs):
+        n = int(np.prod(m["in_size"]))
+        a, b = A[o:o + n], B[o:o + n]
+        items.append(hptt_amd.make_batch_item(a, b, m["in_size"], m["perm"], m["alpha"], m["beta"]))
+        plans.append(hptt_amd.create_plan_checked(a, b, m["in_size"], m["perm"], m["alpha"],
+                                                  m["beta"], stream=stream))
+    exec_fn = hptt_amd.lib().hpttx_plan_exec
+    handles = [p._h for p in plans]
+
+    def run_seq():
+        for h in handles:
+            if exec_fn(h) != 0:
+                raise RuntimeError(hptt_amd.last_error())
+
+    run_seq()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    seq_ms = []
+    for _ in range(seq_reps):
+        e0.record(stream)
+        run_seq()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        seq_ms.append(e0.elapsed_time(e1))
+    B.copy_(B0)
+    torch.cuda.synchronize()
+    run_seq()
+    torch.cuda.synchronize()
+    Bseq = B.clone()
+    srv = hptt_amd.Server(slots=slots, max_batch=max_batch)
+    arr = (hptt_amd.BatchItem * len(items))(*items)   # the request array a client hands over
+    walls, launches = [], []
+    same = True
+    iv = torch.int32 if esz == 4 else torch.int64
+    for k in range(passes):
+        B.copy_(B0)
+        torch.cuda.synchronize()
+        n0 = srv.stats().launches
+        t0 = srv.now()
+        tickets = srv.submit_many(arr)
+        srv.drain()
+        t_last = 0.0
+        for t in tickets:
+            c = srv.wait(t, 10.0)
+            if c.state != 2:
+                raise RuntimeError("served request failed (%d)" % c.status)
+            t_last = max(t_last, c.t_done)
+        walls.append(t_last - t0)
+        launches.append(srv.stats().launches - n0)
+        torch.cuda.synchronize()
+        same = same and bool(torch.equal(B.view(iv), Bseq.view(iv)))
+    srv.close()
+    seq = float(np.median(seq_ms)) * 1e-3
+    warm = float(np.median(walls[1:])) if passes > 1 else walls[0]
+    res = {"tensors": len(man), "sequential_ms": round(seq * 1e3, 3),
+           "served_cold_ms": round(walls[0] * 1e3, 3), "served_warm_ms": round(warm * 1e3, 3),
+           "sequential_transposes_per_s": round(len(man) / seq, 1),
+           "served_transposes_per_s": round(len(man) / warm, 1),
+           "speedup_warm": round(seq / warm, 2), "speedup_cold": round(seq / walls[0], 2),
+           "launches_per_pass": launches, "max_batch": max_batch, "slots": slots,
+           "identical_to_sequential": same,
+           "note": "one hpttx_server_submit_many call per pass, closed loop; wall = host clock "
+                   "from before the call to the last completion; cold = plans made in the "
+                   "call, warm = plans cached"}
+    del srv, plans, A, B, B0, Bseq
+    torch.cuda.empty_cache()
+    return res
+
+
 def time_sweep(reps=10):
     """The reference benchmark's 102 workloads (fp32, alpha=2.3, beta=4.2,
     test/benchmark/inc/hptt/benchmark/benchmark_trans.h:24-140): effective
@@ -571,6 +663,10 @@ def main():
     if args.workload == "serving":
         res = time_serving()
         print(json.dumps({"workload": "serving", **res}), flush=True)
+        return
+    if args.workload.startswith("serving-manifest-"):
+        res = time_serving_manifest({"f32": "float32", "f64": "float64"}[args.workload.split("-")[2]])
+        print(json.dumps({"workload": args.workload, **res}), flush=True)
         return
     if args.workload.startswith("batched-"):
         res = time_batched({"f32": "float32", "f64": "float64"}[args.workload.split("-")[1]],

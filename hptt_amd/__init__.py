@@ -497,6 +497,9 @@ def _bind_server(L):
     L.hpttx_server_create.argtypes = [ctypes.POINTER(ServerConfigC), ctypes.POINTER(vp)]
     L.hpttx_server_submit.restype = ctypes.c_int
     L.hpttx_server_submit.argtypes = [vp, ctypes.POINTER(BatchItem), ctypes.POINTER(u64)]
+    L.hpttx_server_submit_many.restype = ctypes.c_int
+    L.hpttx_server_submit_many.argtypes = [vp, u32, ctypes.POINTER(BatchItem),
+                                           ctypes.POINTER(u64), ctypes.POINTER(u32)]
     L.hpttx_server_poll.restype = ctypes.c_int
     L.hpttx_server_poll.argtypes = [vp, ctypes.POINTER(Completion), u32, ctypes.POINTER(u32)]
     L.hpttx_server_wait.restype = ctypes.c_int
@@ -530,6 +533,17 @@ class Server:
         t = ctypes.c_uint64()
         _check(lib().hpttx_server_submit(self._h, ctypes.byref(item), ctypes.byref(t)))
         return t.value
+
+    def submit_many(self, items) -> list:
+        """Submit a sequence of BatchItems, or a ctypes array of them (built
+        once and reused), in one call (planned, then queued together);
+        returns their tickets."""
+        n = len(items)
+        arr = items if isinstance(items, ctypes.Array) else (BatchItem * n)(*items)
+        tickets = (ctypes.c_uint64 * max(1, n))()
+        queued = ctypes.c_uint32()
+        _check(lib().hpttx_server_submit_many(self._h, n, arr, tickets, ctypes.byref(queued)))
+        return [tickets[i] for i in range(n)]
 
     def poll(self, max_n: int = 256):
         buf = (Completion * max_n)()

@@ -209,9 +209,19 @@ static int plan_rows(const Problem &p, int rowdim, const TuneParams &tp,
   }
   lp.vr_shape = lp.vw_shape = v;
   // A block tile covers ~block_elems elements: one segment of RB rows.
+  // Rows longer than that are cut into equal segments (a multiple of a
+  // 128-byte line where that keeps the segment count; else of the vector):
+  // 2144-element fp32 rows went from segments of 2048 + 96 to 1088 + 1056.
   const uint32_t be = std::max<uint32_t>(tp.block_elems, kBlock);
   uint32_t seg = (uint32_t)std::min<uint64_t>(L, be);
-  if (seg < L) seg = std::max<uint32_t>(v, seg - seg % vcap);
+  if (seg < L) {
+    const uint64_t nseg = (L + be - 1) / be;
+    const uint64_t even = (L + nseg - 1) / nseg;
+    const uint64_t line = std::max<uint64_t>(vcap, 128 / p.esize);
+    uint64_t s2 = (even + line - 1) / line * line;
+    if ((L + s2 - 1) / s2 != nseg) s2 = (even + vcap - 1) / vcap * vcap;
+    seg = (uint32_t)std::max<uint64_t>(v, std::min<uint64_t>(s2, L));
+  }
   uint32_t rb = std::max<uint32_t>(1, be / std::max<uint32_t>(seg, 1));
   rb = std::min<uint32_t>(rb, 1024);
   rb = (uint32_t)std::min<uint64_t>(rb, nrows);

@@ -5,7 +5,6 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
-#include <sstream>
 
 #include "device.h"
 
@@ -135,6 +134,16 @@ int Server::start(const ServerConfig &cfg, std::string &err) {
     if (e == hipSuccess) e = hipMalloc(&s.d_begins, sizeof(uint32_t) * cfg_.max_batch);
     if (e == hipSuccess) e = hipHostMalloc(&s.h_ent, sizeof(BatchEntry) * cfg_.max_batch, hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc(&s.h_begins, sizeof(uint32_t) * cfg_.max_batch, hipHostMallocDefault);
+    for (size_t i = 0; i + 1 < kBatchLanes && e == hipSuccess; ++i) {
+      hipStream_t ss = nullptr;
+      e = hipStreamCreateWithFlags(&ss, hipStreamNonBlocking);
+      if (e == hipSuccess) s.side.push_back(ss);
+    }
+    for (size_t i = 0; i < kBatchLanes && e == hipSuccess; ++i) {
+      hipEvent_t ev = nullptr;
+      e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (e == hipSuccess) s.fork.push_back(ev);
+    }
     if (e != hipSuccess) {
       err = std::string("server slot setup: ") + hipGetErrorString(e);
       stop();
@@ -176,6 +185,11 @@ void Server::stop() {
     if (s.d_begins) (void)hipFree(s.d_begins);
     if (s.h_ent) (void)hipHostFree(s.h_ent);
     if (s.h_begins) (void)hipHostFree(s.h_begins);
+    for (hipStream_t ss : s.side) {
+      (void)hipStreamSynchronize(ss);
+      (void)hipStreamDestroy(ss);
+    }
+    for (hipEvent_t ev : s.fork) (void)hipEventDestroy(ev);
     if (s.done) (void)hipEventDestroy(s.done);
     if (s.stream) (void)hipStreamDestroy(s.stream);
     s = Slot();
@@ -188,7 +202,8 @@ void Server::stop() {
   cv_done_.notify_all();
 }
 
-int Server::plan_cached(const BatchItemSpec &spec, PlannedItem &q, std::string &err) {
+int Server::plan_cached(const BatchItemSpec &spec, std::shared_ptr<const PlannedItem> &q,
+                        std::string &err) {
   // Plans depend on shape, perm, padding, update / complex-coefficient flags
   // and the pointers' 16- and 128-byte alignment (vectors; streaming), not
   // on the coefficient values.
@@ -196,14 +211,22 @@ int Server::plan_cached(const BatchItemSpec &spec, PlannedItem &q, std::string &
   const bool upd = cc ? (std::fabs(spec.coef.br) >= 1e-16 || std::fabs(spec.coef.bi) >= 1e-16)
                       : std::fabs(spec.coef.br) >= 1e-16;
   const uint32_t es = dtype_esize(spec.dtype);
-  std::ostringstream os;
-  os << spec.dtype << '|' << spec.rank << '|' << upd << '|' << cc << '|'
-     << (es ? align_vec(spec.A, es, 16 / es) : 0) << '|' << (es ? align_vec(spec.B, es, 16 / es) : 0)
-     << '|' << ((((uintptr_t)spec.A | (uintptr_t)spec.B) % 128) == 0);
-  for (uint32_t d = 0; d < spec.rank && d < (uint32_t)kMaxRank; ++d)
-    os << '|' << spec.in_size[d] << ',' << spec.perm[d] << ',' << spec.in_outer[d] << ','
-       << spec.out_outer[d];
-  const std::string key = os.str();
+  // A binary key (this runs once per request, so no formatting).
+  std::string key;
+  key.reserve(20 + 16 * (size_t)kMaxRank);
+  auto put = [&key](uint32_t v) { key.append(reinterpret_cast<const char *>(&v), sizeof v); };
+  put((uint32_t)spec.dtype);
+  put(spec.rank);
+  put((upd ? 1u : 0u) | (cc ? 2u : 0u) |
+      ((((uintptr_t)spec.A | (uintptr_t)spec.B) % 128) == 0 ? 4u : 0u));
+  put(es ? align_vec(spec.A, es, 16 / es) : 0);
+  put(es ? align_vec(spec.B, es, 16 / es) : 0);
+  for (uint32_t d = 0; d < spec.rank && d < (uint32_t)kMaxRank; ++d) {
+    put(spec.in_size[d]);
+    put(spec.perm[d]);
+    put(spec.in_outer[d]);
+    put(spec.out_outer[d]);
+  }
   {
     std::lock_guard<std::mutex> lk(plan_mu_);
     auto it = plans_.find(key);
@@ -212,52 +235,100 @@ int Server::plan_cached(const BatchItemSpec &spec, PlannedItem &q, std::string &
       return kOk;
     }
   }
-  int rc = plan_item(spec, TuneParams{}, q, err);
+  auto fresh = std::make_shared<PlannedItem>();
+  int rc = plan_item(spec, TuneParams{}, *fresh, err);
   if (rc != kOk) return rc;
+  q = fresh;
   std::lock_guard<std::mutex> lk(plan_mu_);
-  if (plans_.size() > 4096) plans_.clear();
+  if (plans_.size() >= kPlanCacheEntries) plans_.clear();   // requests keep theirs
   plans_[key] = q;
   return kOk;
 }
 
-int Server::submit(const BatchItemSpec &spec_in, bool server_out, uint64_t &ticket, std::string &err) {
+int Server::make_request(const BatchItemSpec &spec_in, bool server_out, Request *&out,
+                         std::string &err) {
+  out = nullptr;
   BatchItemSpec spec = spec_in;
   if (server_out) {
     if (!arena_.capacity()) { err = "server has no output arena"; return kEInval; }
     spec.B = arena_.base();   // stands in for a 256-byte-aligned arena slot while planning
   }
-  PlannedItem q;
+  std::shared_ptr<const PlannedItem> q;
   int rc = plan_cached(spec, q, err);
   if (rc != kOk) return rc;
-  if (server_out && q.update) { err = "server-allocated outputs need beta below 1e-16"; return kEInval; }
+  if (server_out && q->update) { err = "server-allocated outputs need beta below 1e-16"; return kEInval; }
   Request *r = new Request();
   r->spec = spec;
   r->plan = q;
-  r->bytes = q.p.N * q.p.esize * (uint64_t)(q.update ? 3 : 2);
+  r->bytes = q->p.N * q->p.esize * (uint64_t)(q->update ? 3 : 2);
   r->arena_out = server_out;
-  r->arena_len = server_out ? q.p.spanB * q.p.esize : 0;
+  r->arena_len = server_out ? q->p.spanB * q->p.esize : 0;
   if (server_out && round_up(r->arena_len, kGranule) > arena_.capacity()) {
     delete r;
     err = "output larger than the server's arena";
     return kENoMem;
   }
+  out = r;
+  return kOk;
+}
+
+// Queue planned requests (mu_ held); false once the server has stopped.
+bool Server::enqueue_locked(Request *r, uint64_t &ticket) {
+  if (stop_) return false;
+  r->ticket = next_ticket_++;
+  r->t_submit = now();
+  if (stats_.first_submit < 0) stats_.first_submit = r->t_submit;
+  stats_.submitted++;
+  all_[r->ticket] = r;
+  pending_.push_back(r);
+  ticket = r->ticket;
+  return true;
+}
+
+int Server::submit(const BatchItemSpec &spec, bool server_out, uint64_t &ticket, std::string &err) {
+  Request *r = nullptr;
+  int rc = make_request(spec, server_out, r, err);
+  if (rc != kOk) return rc;
   {
     std::lock_guard<std::mutex> lk(mu_);
-    if (stop_) {
+    if (!enqueue_locked(r, ticket)) {
       delete r;
       err = "server stopped";
       return kEInval;
     }
-    r->ticket = next_ticket_++;
-    r->t_submit = now();
-    if (stats_.first_submit < 0) stats_.first_submit = r->t_submit;
-    stats_.submitted++;
-    all_[r->ticket] = r;
-    pending_.push_back(r);
-    ticket = r->ticket;
   }
   cv_dispatch_.notify_one();
   return kOk;
+}
+
+int Server::submit_many(const BatchItemSpec *specs, const bool *server_out, size_t count,
+                        uint64_t *tickets, size_t &queued, std::string &err) {
+  queued = 0;
+  std::vector<Request *> reqs;
+  reqs.reserve(count);
+  int rc = kOk;
+  for (size_t i = 0; i < count; ++i) {
+    Request *r = nullptr;
+    rc = make_request(specs[i], server_out[i], r, err);
+    if (rc != kOk) break;
+    reqs.push_back(r);
+  }
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    for (Request *r : reqs) {
+      if (!enqueue_locked(r, tickets[queued])) break;
+      ++queued;
+    }
+  }
+  if (queued < reqs.size()) {
+    for (size_t i = queued; i < reqs.size(); ++i) delete reqs[i];
+    if (rc == kOk) {
+      err = "server stopped";
+      rc = kEInval;
+    }
+  }
+  if (queued) cv_dispatch_.notify_one();
+  return rc;
 }
 
 uint64_t Server::live_budget_locked() {
@@ -346,7 +417,7 @@ void Server::dispatch_loop() {
     const bool merged = batch_merged_default(false);
     for (Request *r : batch) {
       BatchEntry probe;
-      make_entry(r->plan, r->spec, probe);
+      make_entry(*r->plan, r->spec, probe);
       r->variant = batch_group_key(probe.variant, merged);
       r->lds_class = merged ? 0u : batch_lds_class(probe.lds_bytes);
     }
@@ -354,7 +425,7 @@ void Server::dispatch_loop() {
       if (a->spec.dtype != b->spec.dtype) return a->spec.dtype < b->spec.dtype;
       if (a->variant != b->variant) return a->variant < b->variant;
       if (a->lds_class != b->lds_class) return a->lds_class < b->lds_class;
-      return a->plan.p.N > b->plan.p.N;
+      return a->plan->p.N > b->plan->p.N;
     });
     struct Group {
       int dtype;
@@ -369,7 +440,7 @@ void Server::dispatch_loop() {
         groups.push_back(Group{r->spec.dtype, r->variant, r->lds_class, i, 0, 0, 0});
       Group &g = groups.back();
       BatchEntry &e = slot->h_ent[i];
-      make_entry(r->plan, r->spec, e);
+      make_entry(*r->plan, r->spec, e);
       if ((uint64_t)g.total + e.ntiles > 0xFFFFFFFFull) he = hipErrorInvalidValue;
       e.tile_begin = g.total;
       slot->h_begins[i] = g.total;
@@ -386,12 +457,37 @@ void Server::dispatch_loop() {
       he = hipMemcpyAsync(slot->d_begins, slot->h_begins, sizeof(uint32_t) * n,
                           hipMemcpyHostToDevice, slot->stream);
     if (groups.size() > kMaxSlotGroups) he = hipErrorInvalidValue;
-    for (size_t gi = 0; gi < groups.size() && he == hipSuccess; ++gi) {
+    // Several groups: fork them over the slot's lanes (the groups' tiles,
+    // largest group first, to the lane with the fewest so far) and join the
+    // lanes back onto the slot's stream, as a batch exec does.  The tables
+    // were copied on the slot's stream, before the fork.
+    const size_t lanes = groups.size() > 1 ? 1 + slot->side.size() : 1;
+    if (lanes > 1 && he == hipSuccess) {
+      he = hipEventRecord(slot->fork[0], slot->stream);
+      for (size_t i = 0; i + 1 < lanes && he == hipSuccess; ++i)
+        he = hipStreamWaitEvent(slot->side[i], slot->fork[0], 0);
+    }
+    std::vector<size_t> order(groups.size());
+    for (size_t gi = 0; gi < order.size(); ++gi) order[gi] = gi;
+    std::sort(order.begin(), order.end(),
+              [&](size_t x, size_t y) { return groups[x].total > groups[y].total; });
+    std::vector<uint64_t> load(lanes, 0);
+    for (size_t gi : order) {
+      if (he != hipSuccess) break;
       const Group &g = groups[gi];
+      size_t best = 0;
+      for (size_t l = 1; l < lanes; ++l)
+        if (load[l] < load[best]) best = l;
+      load[best] += g.total;
+      hipStream_t ls = best == 0 ? slot->stream : slot->side[best - 1];
       he = launch_batch_group(g.dtype, g.variant, slot->d_ent + g.first, slot->d_begins + g.first,
                               nullptr, g.count, g.total, g.lds,
                               batch_grid(g.dtype, g.variant, g.lds, g.total, batch_persistent()),
-                              slot->stream);
+                              ls);
+    }
+    for (size_t i = 0; i + 1 < lanes && he == hipSuccess; ++i) {
+      he = hipEventRecord(slot->fork[1 + i], slot->side[i]);
+      if (he == hipSuccess) he = hipStreamWaitEvent(slot->stream, slot->fork[1 + i], 0);
     }
     if (he == hipSuccess) he = hipEventRecord(slot->done, slot->stream);
     lk.lock();
@@ -401,6 +497,7 @@ void Server::dispatch_loop() {
       const std::string why = std::string("launch: ") + hipGetErrorString(he);
       lk.unlock();
       (void)hipStreamSynchronize(slot->stream);
+      for (hipStream_t ss : slot->side) (void)hipStreamSynchronize(ss);
       lk.lock();
       settle_requests(slot->reqs, kEHip, why, now(), stats_, lat_, finished_, arena_);
       progress_++;

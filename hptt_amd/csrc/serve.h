@@ -21,6 +21,7 @@
 #include <cstdint>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -48,7 +49,7 @@ enum ReqState : int { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3 };
 struct Request {
   uint64_t ticket = 0;
   BatchItemSpec spec{};
-  PlannedItem plan;
+  std::shared_ptr<const PlannedItem> plan;   // shared with the plan cache
   uint64_t bytes = 0;            // algorithmic bytes
   uint64_t arena_off = 0;        // output in the arena (when server-allocated)
   uint64_t arena_len = 0;
@@ -81,6 +82,12 @@ class Arena {
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
+  // Side streams of the slot: a launch round's groups are forked over the
+  // slot's stream and these (kBatchLanes in all, as a batch's are) and
+  // joined back before `done`.  fork[0] is recorded on the slot's stream,
+  // fork[1 + i] on side[i].
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> fork;
   BatchEntry *d_ent = nullptr;
   uint32_t *d_begins = nullptr;
   BatchEntry *h_ent = nullptr;     // pinned staging
@@ -113,6 +120,12 @@ class Server {
   int start(const ServerConfig &cfg, std::string &err);
   void stop();
   int submit(const BatchItemSpec &spec, bool server_out, uint64_t &ticket, std::string &err);
+  // Requests planned first, then queued together under one lock with one
+  // wake-up of the dispatcher (which then finds them all pending).  Stops at
+  // the first request that fails to plan: *queued of them are queued, with
+  // their tickets in tickets[0 .. *queued).
+  int submit_many(const BatchItemSpec *specs, const bool *server_out, size_t count,
+                  uint64_t *tickets, size_t &queued, std::string &err);
   // Completed (done or failed) tickets not yet reported by poll.
   size_t poll(Request *out, size_t max);
   bool wait(uint64_t ticket, double timeout_s, Request &out);
@@ -132,7 +145,10 @@ class Server {
   // Budget for the next admission round (mu_ held): the configured one, or
   // a quarter of the HBM free right now.
   uint64_t live_budget_locked();
-  int plan_cached(const BatchItemSpec &spec, PlannedItem &q, std::string &err);
+  int plan_cached(const BatchItemSpec &spec, std::shared_ptr<const PlannedItem> &q,
+                  std::string &err);
+  int make_request(const BatchItemSpec &spec, bool server_out, Request *&out, std::string &err);
+  bool enqueue_locked(Request *r, uint64_t &ticket);
 
   ServerConfig cfg_;
   uint64_t budget_ = 0;
@@ -151,8 +167,10 @@ class Server {
   ServerStats stats_;
   std::vector<double> lat_;
   std::thread dispatcher_, completer_;
+  // Plans by shape (about 1.5 KB each); the cache is emptied when full.
+  static constexpr size_t kPlanCacheEntries = 16384;
   std::mutex plan_mu_;
-  std::unordered_map<std::string, PlannedItem> plans_;
+  std::unordered_map<std::string, std::shared_ptr<const PlannedItem>> plans_;
   int device_ = 0;
 };
 

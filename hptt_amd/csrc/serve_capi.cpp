@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -67,6 +68,29 @@ int hpttx_server_submit(hpttx_server *h, const hpttx_batch_item *req, uint64_t *
   if (rc != HPTTX_OK) return rc;
   std::string err;
   rc = h->s.submit(spec, req->out == nullptr, *ticket, err);
+  if (rc != kOk) set_last_error(err);
+  return rc;
+}
+
+int hpttx_server_submit_many(hpttx_server *h, uint32_t count, const hpttx_batch_item *reqs,
+                             uint64_t *tickets, uint32_t *queued) {
+  if (!h || !queued || (count && (!reqs || !tickets))) {
+    set_last_error("null argument");
+    return HPTTX_EINVAL;
+  }
+  *queued = 0;
+  std::vector<BatchItemSpec> specs(count);
+  std::unique_ptr<bool[]> own(new bool[count ? count : 1]);
+  for (uint32_t i = 0; i < count; ++i) {
+    tickets[i] = 0;
+    own[i] = reqs[i].out == nullptr;
+    int rc = batch_item_to_spec(reqs[i], i, specs[i], true);
+    if (rc != HPTTX_OK) return rc;   // an argument error queues none
+  }
+  std::string err;
+  size_t n = 0;
+  int rc = h->s.submit_many(specs.data(), own.get(), count, tickets, n, err);
+  *queued = (uint32_t)n;
   if (rc != kOk) set_last_error(err);
   return rc;
 }

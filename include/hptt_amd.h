@@ -299,6 +299,16 @@ typedef struct hpttx_server_stats {
 int hpttx_server_create(const hpttx_server_config *config, hpttx_server **server);
 int hpttx_server_submit(hpttx_server *server, const hpttx_batch_item *request,
                         uint64_t *ticket);
+/* Submit `count` requests at once: all are planned, then queued together
+ * (one wake-up of the dispatcher, which then sees them all pending), in
+ * order.  tickets[i] receives request i's ticket (0 if not queued) and
+ * *queued how many were queued.  An argument error queues none; a request
+ * that fails to plan stops the submission there (the requests before it are
+ * queued).  No counterpart in the reference; for callers that hand over many
+ * transpositions at a time (one call instead of one per request). */
+int hpttx_server_submit_many(hpttx_server *server, uint32_t count,
+                             const hpttx_batch_item *requests, uint64_t *tickets,
+                             uint32_t *queued);
 /* Up to max finished requests not reported before; *n receives the count. */
 int hpttx_server_poll(hpttx_server *server, hpttx_completion *out,
                       uint32_t max, uint32_t *n);

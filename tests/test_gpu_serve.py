@@ -87,6 +87,54 @@ def test_served_equals_alone_and_oracle(device):
     srv.close()
 
 
+def test_submit_many_equals_alone_and_oracle(device):
+    """Requests handed over in groups (hpttx_server_submit_many): each result
+    equals its plain plan and the oracle; a rejected request stops the
+    submission at that request."""
+    import torch
+    rng = np.random.default_rng(4096)
+    reqs = _requests(rng, 120)
+    torch.cuda.synchronize()
+    srv = hptt_amd.Server(slots=3, max_batch=256)
+    tickets = []
+    for lo in range(0, len(reqs), 50):
+        tickets += srv.submit_many([_item(r) for r in reqs[lo:lo + 50]])
+    assert len(set(tickets)) == len(reqs) and 0 not in tickets
+    srv.drain()
+    st = srv.stats()
+    assert st.completed == len(reqs) and st.failed == 0
+    for k, r in enumerate(reqs):
+        c = srv.wait(tickets[k], 5.0)
+        assert c.state == 2, (k, c.status)
+        got = host(r["dB"])
+        exp = r["B"].copy()
+        oracle.transpose(r["A"], exp, r["size"], r["perm"], r["alpha"], r["beta"], nthreads=8,
+                         complex_coef=r["cc"])
+        assert oracle.first_mismatch(got, exp) == -1, k
+        assert oracle.first_mismatch(got, _alone(r)) == -1, k
+    # the third request needs a server-allocated output with beta != 0 (no
+    # arena): the first two are queued, the rest are not
+    extra = _requests(rng, 4)
+    items = [_item(r) for r in extra]
+    items[2] = _item(extra[2], out=False)
+    L = hptt_amd.lib()
+    import ctypes
+    arr = (hptt_amd.BatchItem * 4)(*items)
+    tk = (ctypes.c_uint64 * 4)()
+    q = ctypes.c_uint32()
+    rc = L.hpttx_server_submit_many(srv._h, 4, arr, tk, ctypes.byref(q))
+    assert rc != hptt_amd.OK and q.value == 2 and tk[0] and tk[1] and not tk[2] and not tk[3]
+    srv.drain()
+    for i in range(2):
+        c = srv.wait(tk[i], 5.0)
+        assert c.state == 2
+        exp = extra[i]["B"].copy()
+        oracle.transpose(extra[i]["A"], exp, extra[i]["size"], extra[i]["perm"], extra[i]["alpha"],
+                         extra[i]["beta"], complex_coef=extra[i]["cc"])
+        assert oracle.first_mismatch(host(extra[i]["dB"]), exp) == -1, i
+    srv.close()
+
+
 def test_server_allocated_outputs(device):
     import torch
     rng = np.random.default_rng(77)

@@ -135,6 +135,21 @@ def test_emulated_wide_blocks(dt):
     assert rc == 0 and "grid: 165888 x 256 " in d, d
 
 
+@pytest.mark.parametrize("dt", shapes.DTYPES)
+def test_emulated_long_rows_in_segments(dt):
+    # kept rows longer than a block's elements are cut into equal segments
+    # (2144 -> 1088 + 1056 for fp32), exact and unaligned, with and without beta
+    for size, perm in (([2144, 3, 4], [0, 2, 1]), ([4099, 2, 3], [0, 2, 1]), ([2100, 5], [0, 1]),
+                       ([1025, 3, 3], [0, 2, 1])):
+        for be in (0.0, 4.2):
+            for off in (0, 1):
+                for tune in (None, [0, 0, 512, 0], [0, 0, 1000, 0]):
+                    d = _check(dt, size, perm, 2.3, be, tune=tune, offset=off)
+                    assert "wide-copy" in d or "axpby" in d, d
+    rc, err, d = E.plan(np.float32, [2144, 64, 384], [0, 2, 1])
+    assert rc == 0 and "seg=1088 " in d, d
+
+
 def test_emulated_benchmark_shapes_reduced():
     # the benchmark workloads' perms on reduced extents (same fusion classes)
     _check(np.float32, [16, 12, 8, 4, 6, 8], shapes.CONFIG2[1], 2.3, 4.2)
