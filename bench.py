@@ -700,6 +700,14 @@ def main():
                 % (name, secondary[name]["batched_transposes_per_s"],
                    secondary[name]["sequential_transposes_per_s"], secondary[name]["speedup"],
                    secondary[name]["identical_to_sequential"]))
+        for name in ("serving-manifest-f32", "serving-manifest-f64"):
+            secondary[name] = time_serving_manifest(
+                {"f32": "float32", "f64": "float64"}[name.split("-")[2]])
+            log("bench: %s served %s transposes/s (warm plans) vs %s sequential (x%.2f), "
+                "identical=%s" % (name, secondary[name]["served_transposes_per_s"],
+                                  secondary[name]["sequential_transposes_per_s"],
+                                  secondary[name]["speedup_warm"],
+                                  secondary[name]["identical_to_sequential"]))
         secondary["serving"] = time_serving()
         sv = secondary["serving"]
         log("bench: serving closed loop %s/s vs sequential %s/s (x%s); " % (
