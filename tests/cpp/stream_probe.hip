@@ -7,10 +7,13 @@
 // the size given in MiB (default 2048).  Variants: U 16-byte vectors per
 // thread, block size, one-shot grid (one block per chunk of U * block
 // vectors) or a persistent grid-stride loop, non-temporal or default loads
-// and stores.  Also hipMemcpyAsync device to device.  Every access is
-// guarded by idx < n (n = number of 16-byte vectors).  Prints GB/s per
-// variant: bytes moved / mean event-timed duration of 20 back-to-back
-// launches after one warm-up launch.
+// and stores.  Also hipMemcpyAsync device to device.  Every store is
+// guarded by idx < n (n = number of 16-byte vectors); loads are guarded the
+// same way (bf=0), or issued for every slot with out-of-range slots reading
+// vector 0 (bf=1: no branch around a load, so all U loads of a thread are in
+// flight together; under a branch per load the compiler waits for each load
+// before the next).  Prints GB/s per variant: bytes moved / mean event-timed
+// duration of 20 back-to-back launches after one warm-up launch.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -40,7 +43,7 @@ __device__ __forceinline__ void st(f4 *p, f4 v) {
 }
 
 // MODE: 0 read, 1 write, 2 copy, 3 axpby
-template <int MODE, int U, bool NT>
+template <int MODE, int U, bool NT, bool BF>
 __global__ void k_stream(const f4 *__restrict__ a, f4 *__restrict__ b, size_t n, int persistent,
                          float al, float be, float *sink) {
   const size_t per_block = (size_t)blockDim.x * U;
@@ -52,7 +55,11 @@ __global__ void k_stream(const f4 *__restrict__ a, f4 *__restrict__ b, size_t n,
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const size_t i = base + (size_t)u * blockDim.x;
-      if (i < n) {
+      if (BF) {
+        const size_t j = i < n ? i : 0;
+        if (MODE == 0 || MODE == 2 || MODE == 3) x[u] = ld<NT>(a + j);
+        if (MODE == 3) y[u] = ld<NT>(b + j);
+      } else if (i < n) {
         if (MODE == 0 || MODE == 2 || MODE == 3) x[u] = ld<NT>(a + i);
         if (MODE == 3) y[u] = ld<NT>(b + i);
       }
@@ -74,17 +81,17 @@ __global__ void k_stream(const f4 *__restrict__ a, f4 *__restrict__ b, size_t n,
 typedef void (*KFn)(const f4 *, f4 *, size_t, int, float, float, float *);
 
 template <int MODE, int U, bool NT>
-KFn pick() {
-  return k_stream<MODE, U, NT>;
+KFn pick(bool bf) {
+  return bf ? k_stream<MODE, U, NT, true> : k_stream<MODE, U, NT, false>;
 }
 
-static KFn table(int mode, int u, bool nt) {
+static KFn table(int mode, int u, bool nt, bool bf) {
 #define P(M)                                                                         \
   if (mode == M) {                                                                   \
-    if (u == 1) return nt ? pick<M, 1, true>() : pick<M, 1, false>();                \
-    if (u == 2) return nt ? pick<M, 2, true>() : pick<M, 2, false>();                \
-    if (u == 4) return nt ? pick<M, 4, true>() : pick<M, 4, false>();                \
-    if (u == 8) return nt ? pick<M, 8, true>() : pick<M, 8, false>();                \
+    if (u == 1) return nt ? pick<M, 1, true>(bf) : pick<M, 1, false>(bf);            \
+    if (u == 2) return nt ? pick<M, 2, true>(bf) : pick<M, 2, false>(bf);            \
+    if (u == 4) return nt ? pick<M, 4, true>(bf) : pick<M, 4, false>(bf);            \
+    if (u == 8) return nt ? pick<M, 8, true>(bf) : pick<M, 8, false>(bf);            \
   }
   P(0) P(1) P(2) P(3)
 #undef P
@@ -128,11 +135,13 @@ int main(int argc, char **argv) {
   const double mult[] = {1, 1, 2, 3};
   for (int mode = 0; mode < 4; ++mode)
     for (int nt = 0; nt < 2; ++nt)
+      for (int bf = 0; bf < 2; ++bf)
       for (int u : {1, 2, 4, 8})
         for (int blk : {256, 512, 1024})
           for (int pers : {0, 1}) {
             if (u * blk > 4096 && u > 2) continue;
-            KFn k = table(mode, u, nt);
+            if (bf && u == 1) continue;   // one load per thread: no branch to remove
+            KFn k = table(mode, u, nt, bf);
             const size_t per_block = (size_t)blk * u;
             const size_t nchunks = (n + per_block - 1) / per_block;
             const size_t grid = pers ? (size_t)ncu * (2048 / blk) * 2 : nchunks;
@@ -141,8 +150,8 @@ int main(int argc, char **argv) {
                                  0.5f, sink);
               CK(hipGetLastError());
             });
-            std::printf("%-6s nt=%d U=%d block=%4d %-10s %.1f GB/s\n", names[mode], nt, u, blk,
-                        pers ? "persistent" : "oneshot", mult[mode] * bytes / (ms * 1e-3) / 1e9);
+            std::printf("%-6s nt=%d bf=%d U=%d block=%4d %-10s %.1f GB/s\n", names[mode], nt, bf, u,
+                        blk, pers ? "persistent" : "oneshot", mult[mode] * bytes / (ms * 1e-3) / 1e9);
             std::fflush(stdout);
           }
   CK(hipFree(a));
