@@ -170,7 +170,7 @@ void put(char *buf, size_t len, const std::string &s) {
 extern "C" {
 
 // Plans like the library (same planner, same pointer-alignment rule) and
-// emulates the launch.  tune: NULL or {run_r, run_c, block_elems, wide_min}.
+// emulates the launch.  tune: NULL or {run_r, run_c, block_elems, wide_min, stream}.
 // Returns 0 on success, the planner's status on a planning error, 90 on an
 // out-of-bounds access, 91 when an element of B inside the tensor is not
 // written exactly once or padding is written.
@@ -184,7 +184,7 @@ int emu_transpose(int dtype, const void *A, uint64_t nA, void *B, uint64_t nB, u
   int rc = build_problem(dtype, rank, in_size, perm, in_outer, out_outer, p, err);
   if (rc) { put(errbuf, errlen, err); return rc; }
   TuneParams tp{};
-  if (tune) tp = TuneParams{tune[0], tune[1], tune[2], tune[3], 0};
+  if (tune) tp = TuneParams{tune[0], tune[1], tune[2], tune[3], tune[4]};
   LaunchPlan lp;
   // explicit tile parameters are taken as given (no small-problem shrink)
   rc = plan_launch(p, tp, lp, err, tune ? 0 : kMinPlanTiles);
@@ -260,7 +260,7 @@ int emu_plan(int dtype, uint32_t rank, const uint32_t *in_size, const uint32_t *
   int rc = build_problem(dtype, rank, in_size, perm, in_outer, out_outer, p, err);
   if (rc) { put(errbuf, errlen, err); return rc; }
   TuneParams tp{};
-  if (tune) tp = TuneParams{tune[0], tune[1], tune[2], tune[3], 0};
+  if (tune) tp = TuneParams{tune[0], tune[1], tune[2], tune[3], tune[4]};
   LaunchPlan lp;
   rc = plan_launch(p, tp, lp, err, kMinPlanTiles);
   if (rc) { put(errbuf, errlen, err); return rc; }

@@ -48,6 +48,14 @@ def _u32(seq):
     return (ctypes.c_uint32 * len(seq))(*[int(v) for v in seq])
 
 
+def _tune(seq):
+    """Tune parameters (run_r, run_c, block_elems, wide_min_bytes[, stream]),
+    passed to the emulator as five values."""
+    if seq is None:
+        return None
+    return _u32(list(seq) + [0] * (5 - len(seq)))
+
+
 def transpose(A, B, in_size, perm, alpha, beta, in_outer=None, out_outer=None,
               tune=None, complex_coef=False, force_scalar=False):
     """Emulate the library's launch on host arrays A, B (flat numpy arrays).
@@ -59,7 +67,7 @@ def transpose(A, B, in_size, perm, alpha, beta, in_outer=None, out_outer=None,
         CODE[A.dtype], A.ctypes.data, A.size, B.ctypes.data, B.size, len(perm),
         _u32(in_size), _u32(perm), a.real, a.imag if complex_coef else 0.0,
         b.real, b.imag if complex_coef else 0.0, 1 if complex_coef else 0,
-        _u32(in_outer), _u32(out_outer), _u32(tune), 1 if force_scalar else 0,
+        _u32(in_outer), _u32(out_outer), _tune(tune), 1 if force_scalar else 0,
         err, 512, desc, 2048)
     return rc, err.value.decode(), desc.value.decode()
 
@@ -68,5 +76,5 @@ def plan(dtype, in_size, perm, in_outer=None, out_outer=None, tune=None):
     err = ctypes.create_string_buffer(512)
     desc = ctypes.create_string_buffer(2048)
     rc = lib().emu_plan(CODE[np.dtype(dtype)], len(perm), _u32(in_size), _u32(perm),
-                        _u32(in_outer), _u32(out_outer), _u32(tune), err, 512, desc, 2048)
+                        _u32(in_outer), _u32(out_outer), _tune(tune), err, 512, desc, 2048)
     return rc, err.value.decode(), desc.value.decode()

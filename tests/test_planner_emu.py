@@ -172,6 +172,36 @@ def test_plans_for_benchmark_shapes():
     assert rc == 0 and "path: tile\n" in d, d
 
 
+def test_streaming_and_tile_order_rules():
+    """Problems beyond the last-level cache: XCD-aware tile order always;
+    non-temporal loads and stores where every run covers whole 128-byte
+    lines, loads only (4-byte elements) or none (wider) where some run does
+    not; problems the cache holds stream nothing and keep the plain order."""
+    cases = [
+        (np.float32, shapes.CONFIG2, "on"),
+        (np.float64, shapes.CONFIG2, "on"),
+        (np.complex128, shapes.CONFIG3_TILE, "on"),
+        (np.complex128, ([64, 128, 128, 64], [3, 2, 1, 0]), "on"),
+        (np.float32, ([97, 97, 97, 97], [3, 2, 1, 0]), "loads"),
+        (np.float32, ([2320, 384, 59], [1, 0, 2]), "loads"),
+        (np.complex128, ([97, 97, 97, 97], [3, 2, 1, 0]), "off"),
+        (np.float64, ([100, 100, 100, 100], [3, 2, 1, 0]), "off"),
+    ]
+    for dt, (size, perm), mode in cases:
+        rc, err, d = E.plan(dt, size, perm)
+        assert rc == 0 and "path: tile\n" in d, d
+        assert ("streaming: %s " % mode) in d and "xcd-order" in d, (size, perm, d)
+    rc, err, d = E.plan(np.float32, *shapes.CONFIG1)
+    assert rc == 0 and "streaming: off " in d and "xcd-order" not in d, d
+    # kept rows stream loads and stores alike
+    rc, err, d = E.plan(np.complex128, *shapes.CONFIG3_WIDE)
+    assert rc == 0 and "streaming: on " in d, d
+    # a tuned choice is kept: loads only (3) and none (2)
+    for st, mode in ((3, "loads"), (2, "off"), (1, "on")):
+        rc, err, d = E.plan(np.float32, *shapes.CONFIG2, tune=[0, 0, 0, 0, st])
+        assert rc == 0 and ("streaming: %s " % mode) in d, (st, d)
+
+
 def test_fusion_and_validation():
     # adjacent ascending pairs fuse (reference parameter_trans.tcc:235-250)
     rc, err, d = E.plan(np.float32, [4, 5, 6, 7], [2, 3, 0, 1])
