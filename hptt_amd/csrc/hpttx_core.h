@@ -405,19 +405,7 @@ constexpr int unroll_for() {
        : (64 / (int)(V * sizeof(T))) > CAP ? CAP : (64 / (int)(V * sizeof(T)));
 }
 
-// Load the n <= V elements of a partial vector one by one (the rest of `out`
-// is left zero and never stored to B).
-template <typename T, int V, bool NT>
-HPTTX_HD void load_part(const HPTTX_GLOBAL T *p, uint32_t n, T (&out)[V]) {
-#pragma unroll
-  for (int j = 0; j < V; ++j) {
-    T one[1];
-    __builtin_memset(&one[0], 0, sizeof(T));
-    if ((uint32_t)j < n) gload<T, 1, NT>(p + j, one);
-    out[j] = one[0];
-  }
-}
-
+// Store the first n <= V elements of a partial vector one by one.
 template <typename T, int V, bool NT>
 HPTTX_HD void store_part(HPTTX_GLOBAL T *p, uint32_t n, const T (&in)[V]) {
 #pragma unroll
