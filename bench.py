@@ -335,11 +335,14 @@ def time_serving_manifest(dtype_name, passes=3, seq_reps=3, max_batch=4096, slot
     return res
 
 
-def time_sweep(reps=10):
+def time_sweep(reps=10, tuned=True):
     """The reference benchmark's 102 workloads (fp32, alpha=2.3, beta=4.2,
     test/benchmark/inc/hptt/benchmark/benchmark_trans.h:24-140): effective
     GB/s of plain plan->exec() per shape (median of reps event-timed execs),
-    with the minimum and median over shapes and the slowest shapes."""
+    with the minimum and median over shapes and the slowest shapes, for model
+    plans (tuning_timeout=0) and, with tuned, for plans autotuned exhaustively
+    (tuning_timeout=-1, which is how the reference's benchmark creates its
+    plans, test/benchmark/inc/hptt/perf-test/test_perf_util.tcc:22-24)."""
     import torch
 
     import hptt_amd
@@ -352,12 +355,8 @@ def time_sweep(reps=10):
     Bbuf = torch.empty(nmax, device="cuda").uniform_(-1, 1, generator=g)
     stream = torch.cuda.Stream()
     torch.cuda.synchronize()
-    rows = {}
-    for name, size, perm in cases:
-        n = int(np.prod(size))
-        plan = hptt_amd.create_plan_checked(Abuf[:n], Bbuf[:n], size, perm, REFERENCE_SWEEP_ALPHA,
-                                            REFERENCE_SWEEP_BETA, stream=stream)
-        info = plan.info()
+
+    def rate(plan, info):
         plan.exec()
         evs = []
         for _ in range(reps):
@@ -369,19 +368,48 @@ def time_sweep(reps=10):
             evs.append((a, b))
         torch.cuda.synchronize()
         ms = float(np.median([a.elapsed_time(b) for a, b in evs]))
-        gbps = info.algorithmic_bytes / (ms * 1e-3) / 1e9
+        return info.algorithmic_bytes / (ms * 1e-3) / 1e9
+
+    def summary(rates):
+        rates = sorted(rates)
+        return {"min_gbps": rates[0], "median_gbps": float(np.median(rates)),
+                "max_gbps": rates[-1], "min_pct": round(100 * rates[0] / PEAK_GBPS, 1),
+                "median_pct": round(100 * float(np.median(rates)) / PEAK_GBPS, 1)}
+
+    rows = {}
+    for name, size, perm in cases:
+        n = int(np.prod(size))
+        plan = hptt_amd.create_plan_checked(Abuf[:n], Bbuf[:n], size, perm, REFERENCE_SWEEP_ALPHA,
+                                            REFERENCE_SWEEP_BETA, stream=stream)
+        info = plan.info()
+        gbps = rate(plan, info)
         rows[name] = {"size": size, "perm": perm, "gbps": round(gbps, 1),
                       "pct_hbm_peak": round(100 * gbps / PEAK_GBPS, 1),
                       "path": hptt_amd.PATHS[info.path]}
         del plan
-    rates = sorted(r["gbps"] for r in rows.values())
+    # Tuned plans second: a tuned shape's parameters stay in the process's
+    # tune cache and would be picked up by later model plans of the same shape.
+    t_tune = 0.0
+    for name, size, perm in (cases if tuned else []):
+        n = int(np.prod(size))
+        t0 = time.perf_counter()
+        plan = hptt_amd.create_plan_checked(Abuf[:n], Bbuf[:n], size, perm, REFERENCE_SWEEP_ALPHA,
+                                            REFERENCE_SWEEP_BETA, stream=stream,
+                                            tuning_timeout=-1.0)
+        t_tune += time.perf_counter() - t0
+        info = plan.info()
+        rows[name]["gbps_tuned"] = round(rate(plan, info), 1)
+        rows[name]["path_tuned"] = hptt_amd.PATHS[info.path]
+        del plan
     worst = sorted(rows.items(), key=lambda kv: kv[1]["gbps"])[:5]
     del Abuf, Bbuf
     torch.cuda.empty_cache()
-    return {"shapes": len(rows), "min_gbps": rates[0], "median_gbps": float(np.median(rates)),
-            "max_gbps": rates[-1], "min_pct": round(100 * rates[0] / PEAK_GBPS, 1),
-            "median_pct": round(100 * float(np.median(rates)) / PEAK_GBPS, 1),
-            "slowest": [dict(name=k, **v) for k, v in worst], "per_shape": rows}
+    res = {"shapes": len(rows), **summary([r["gbps"] for r in rows.values()]),
+           "slowest": [dict(name=k, **v) for k, v in worst], "per_shape": rows}
+    if tuned:
+        res["tuned"] = {"tuning_timeout": -1.0, "tuning_s_total": round(t_tune, 1),
+                        **summary([r["gbps_tuned"] for r in rows.values()])}
+    return res
 
 
 def cpu_batched_baseline(dtype_name, budget_s=5.0):
@@ -690,8 +718,10 @@ def main():
             log("bench: %s %.1f GB/s" % (name, secondary[name]["gbps"]))
         secondary["reference-sweep"] = time_sweep()
         sw = secondary["reference-sweep"]
-        log("bench: reference sweep (%d shapes) min %.1f median %.1f GB/s; slowest %s"
-            % (sw["shapes"], sw["min_gbps"], sw["median_gbps"], sw["slowest"][0]["name"]))
+        log("bench: reference sweep (%d shapes) min %.1f median %.1f GB/s; slowest %s; "
+            "tuned (%.0f s of tuning) min %.1f median %.1f GB/s"
+            % (sw["shapes"], sw["min_gbps"], sw["median_gbps"], sw["slowest"][0]["name"],
+               sw["tuned"]["tuning_s_total"], sw["tuned"]["min_gbps"], sw["tuned"]["median_gbps"]))
         for name in ("batched-f32", "batched-f64", "batched-f32-one-launch",
                      "batched-f64-one-launch"):
             secondary[name] = time_batched(name.split("-")[1].replace("f", "float"),
