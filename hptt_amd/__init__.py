@@ -536,7 +536,8 @@ class Server:
 
     def submit_many(self, items) -> list:
         """Submit a sequence of BatchItems, or a ctypes array of them (built
-        once and reused), in one call (planned, then queued together);
+        once and reused), in one call (planned and queued in chunks of
+        doubling size, so the first run while the rest are planned);
         returns their tickets."""
         n = len(items)
         arr = items if isinstance(items, ctypes.Array) else (BatchItem * n)(*items)

@@ -304,30 +304,40 @@ int Server::submit(const BatchItemSpec &spec, bool server_out, uint64_t &ticket,
 int Server::submit_many(const BatchItemSpec *specs, const bool *server_out, size_t count,
                         uint64_t *tickets, size_t &queued, std::string &err) {
   queued = 0;
+  // Planned and queued in chunks that double in size (128, 256, ...), the
+  // dispatcher woken after each: the GPU starts on the first chunk while the
+  // rest are planned, and each chunk's kernels outlast the planning of the
+  // next.
   std::vector<Request *> reqs;
-  reqs.reserve(count);
   int rc = kOk;
-  for (size_t i = 0; i < count; ++i) {
-    Request *r = nullptr;
-    rc = make_request(specs[i], server_out[i], r, err);
-    if (rc != kOk) break;
-    reqs.push_back(r);
-  }
-  {
-    std::lock_guard<std::mutex> lk(mu_);
-    for (Request *r : reqs) {
-      if (!enqueue_locked(r, tickets[queued])) break;
-      ++queued;
+  size_t i = 0, chunk = kSubmitFirstChunk;
+  while (i < count && rc == kOk) {
+    reqs.clear();
+    for (; i < count && reqs.size() < chunk; ++i) {
+      Request *r = nullptr;
+      rc = make_request(specs[i], server_out[i], r, err);
+      if (rc != kOk) break;
+      reqs.push_back(r);
     }
-  }
-  if (queued < reqs.size()) {
-    for (size_t i = queued; i < reqs.size(); ++i) delete reqs[i];
-    if (rc == kOk) {
-      err = "server stopped";
-      rc = kEInval;
+    size_t q = 0;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (Request *r : reqs) {
+        if (!enqueue_locked(r, tickets[queued])) break;
+        ++queued;
+        ++q;
+      }
     }
+    if (q) cv_dispatch_.notify_one();
+    if (q < reqs.size()) {
+      for (size_t k = q; k < reqs.size(); ++k) delete reqs[k];
+      if (rc == kOk) {
+        err = "server stopped";
+        rc = kEInval;
+      }
+    }
+    chunk *= 2;
   }
-  if (queued) cv_dispatch_.notify_one();
   return rc;
 }
 

@@ -169,6 +169,8 @@ class Server {
   std::thread dispatcher_, completer_;
   // Plans by shape (about 1.5 KB each); the cache is emptied when full.
   static constexpr size_t kPlanCacheEntries = 16384;
+  // submit_many's first chunk of requests planned before the dispatcher is woken.
+  static constexpr size_t kSubmitFirstChunk = 128;
   std::mutex plan_mu_;
   std::unordered_map<std::string, std::shared_ptr<const PlannedItem>> plans_;
   int device_ = 0;

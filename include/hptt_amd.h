@@ -299,9 +299,10 @@ typedef struct hpttx_server_stats {
 int hpttx_server_create(const hpttx_server_config *config, hpttx_server **server);
 int hpttx_server_submit(hpttx_server *server, const hpttx_batch_item *request,
                         uint64_t *ticket);
-/* Submit `count` requests at once: all are planned, then queued together
- * (one wake-up of the dispatcher, which then sees them all pending), in
- * order.  tickets[i] receives request i's ticket (0 if not queued) and
+/* Submit `count` requests at once, in order: they are planned and queued in
+ * chunks of doubling size (128, 256, ...), the dispatcher woken after each,
+ * so the first launches run while the rest are planned.  tickets[i]
+ * receives request i's ticket (0 if not queued) and
  * *queued how many were queued.  An argument error queues none; a request
  * that fails to plan stops the submission there (the requests before it are
  * queued).  No counterpart in the reference; for callers that hand over many

@@ -135,6 +135,44 @@ def test_submit_many_equals_alone_and_oracle(device):
     srv.close()
 
 
+def test_submit_many_across_chunks(device):
+    """One submission of 420 requests (planned and queued in chunks of 128,
+    256, ...) with a rejected request at index 401, in the third chunk: the
+    401 before it are queued, run and equal the oracle; none after it is
+    queued."""
+    import ctypes
+
+    import torch
+    rng = np.random.default_rng(420)
+    reqs = _requests(rng, 420, max_elems=20000)
+    torch.cuda.synchronize()
+    srv = hptt_amd.Server(slots=3, max_batch=256)
+    items = [_item(r) for r in reqs]
+    items[401] = _item(reqs[401], out=False)   # server output with beta != 0 and no arena
+    assert reqs[401]["beta"] != 0
+    arr = (hptt_amd.BatchItem * len(items))(*items)
+    tk = (ctypes.c_uint64 * len(items))()
+    q = ctypes.c_uint32()
+    rc = hptt_amd.lib().hpttx_server_submit_many(srv._h, len(items), arr, tk, ctypes.byref(q))
+    assert rc != hptt_amd.OK and q.value == 401
+    assert all(tk[i] for i in range(401)) and not any(tk[i] for i in range(401, len(items)))
+    assert len(set(tk[i] for i in range(401))) == 401
+    srv.drain()
+    st = srv.stats()
+    assert st.completed == 401 and st.failed == 0 and st.launches >= 2
+    for k in range(401):
+        r = reqs[k]
+        c = srv.wait(tk[k], 5.0)
+        assert c.state == 2, (k, c.status)
+        exp = r["B"].copy()
+        oracle.transpose(r["A"], exp, r["size"], r["perm"], r["alpha"], r["beta"], nthreads=8,
+                         complex_coef=r["cc"])
+        assert oracle.first_mismatch(host(r["dB"]), exp) == -1, k
+    for k in range(401, len(reqs)):   # not queued: B untouched
+        assert oracle.first_mismatch(host(reqs[k]["dB"]), reqs[k]["B"]) == -1, k
+    srv.close()
+
+
 def test_server_allocated_outputs(device):
     import torch
     rng = np.random.default_rng(77)
