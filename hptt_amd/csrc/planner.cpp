@@ -349,7 +349,7 @@ static int plan_tile(const Problem &p, bool shared, const TuneParams &tp, uint32
   // need no partial vector even where the runs are unaligned).
   auto cdiv = [](uint64_t a, uint64_t b) { return (a + b - 1) / b; };
   const bool beyond_llc = p.N * p.esize * 2 > kLastLevelCacheBytes;
-  auto balance = [&](uint64_t V, uint64_t T, uint32_t vec) {
+  auto balance = [&](uint64_t V, uint64_t T, uint32_t vec, bool is_r) {
     const uint64_t n = cdiv(V, T);
     uint64_t b = cdiv(V, n);
     // vector granularity in index units: runs are index*VS elements
@@ -372,6 +372,20 @@ static int plan_tile(const Problem &p, bool shared, const TuneParams &tp, uint32
       const uint64_t hi = cdiv(b, line) * line;
       if (lo <= cap_r && cdiv(V, lo) == n) b = lo;
       else if (hi <= cap_r && cdiv(V, hi) == n) b = hi;
+      // An R extent of at most 4 tiles that the tile does not divide (96
+      // fp32 elements as 64 + 32): equal runs of whole lines that divide it
+      // (3 x 32) where there are such, so that no block gets a short last
+      // tile.  Measured on MI355X: the reference's 27 shapes with R = 96 and
+      // 4 with R = 160 (fp32) +0.1% to +8.3%, 5% on average; longer extents
+      // (352 to 2400, 5.5 to 37.5 tiles) -11% to +7%, and the same rule on C
+      // +1% at most, so neither is split.
+      if (is_r && V % b != 0 && cdiv(V, b) <= 4) {
+        for (uint64_t d = b / line * line; d >= line; d -= line)
+          if (V % d == 0) {
+            b = d;
+            break;
+          }
+      }
     }
     return std::max<uint64_t>(1, b);
   };
@@ -387,8 +401,8 @@ static int plan_tile(const Problem &p, bool shared, const TuneParams &tp, uint32
       if (grow_r) TR = std::min<uint64_t>(VR, TR * 2);
       else TC = std::min<uint64_t>(VC, TC * 2);
     }
-    TR = balance(VR, TR, vcap);
-    TC = balance(VC, TC, vcap);
+    TR = balance(VR, TR, vcap, true);
+    TC = balance(VC, TC, vcap, false);
     const uint64_t per_o = TR * VS * TC;
     TO = std::max<uint64_t>(1, block / std::max<uint64_t>(per_o, 1));
     TO = std::min<uint64_t>(std::min<uint64_t>(TO, VO), 512);

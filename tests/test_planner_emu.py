@@ -202,6 +202,20 @@ def test_streaming_and_tile_order_rules():
         assert rc == 0 and ("streaming: %s " % mode) in d, (st, d)
 
 
+def test_short_r_extents_split_evenly():
+    """Beyond the last-level cache, an R extent of at most 4 tiles that the
+    tile does not divide is cut into equal runs of whole lines (96 fp32 as
+    3 x 32, 160 as 5 x 32); longer extents keep the model's tile (352 as
+    5 x 64 + 32), and so do extents the tile divides (the 6-D workload's
+    384)."""
+    for size, perm, r in [([96, 75, 75, 96], [3, 2, 1, 0], "R=96/32 "),
+                          ([32, 5, 15, 112, 15, 15], [3, 2, 0, 5, 1, 4], "R=160/32 "),
+                          ([352, 4, 48, 28, 28], [2, 0, 4, 1, 3], "R=352/64 "),
+                          (shapes.CONFIG2[0], shapes.CONFIG2[1], "R=384/64 ")]:
+        rc, err, d = E.plan(np.float32, size, perm)
+        assert rc == 0 and r in d, (size, perm, d)
+
+
 def test_fusion_and_validation():
     # adjacent ascending pairs fuse (reference parameter_trans.tcc:235-250)
     rc, err, d = E.plan(np.float32, [4, 5, 6, 7], [2, 3, 0, 1])
