@@ -376,7 +376,9 @@ static int plan_tile(const Problem &p, bool shared, const TuneParams &tp, uint32
       // fp32 elements as 64 + 32): equal runs of whole lines that divide it
       // (3 x 32) where there are such, so that no block gets a short last
       // tile.  Measured on MI355X: the reference's 27 shapes with R = 96 and
-      // 4 with R = 160 (fp32) +0.1% to +8.3%, 5% on average; longer extents
+      // 4 with R = 160 (fp32) +0.1% to +8.3%, 5% on average; 8 fp64, c64 and
+      // c128 reversals and perms with R of 1.5 to 2.5 tiles (48 and 80 fp64
+      // as 16s, 24 and 40 c128 as 8s) +4% to +12%; longer extents
       // (352 to 2400, 5.5 to 37.5 tiles) -11% to +7%, and the same rule on C
       // +1% at most, so neither is split.
       if (is_r && V % b != 0 && cdiv(V, b) <= 4) {
