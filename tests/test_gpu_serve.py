@@ -173,6 +173,45 @@ def test_submit_many_across_chunks(device):
     srv.close()
 
 
+def test_submit_many_from_two_threads(device):
+    """Two threads hand over 300 requests each in one submit_many call at the
+    same time: their chunks interleave in the queue, every ticket is unique,
+    each thread's tickets rise in its own order, and every result equals the
+    oracle."""
+    import torch
+    rng = np.random.default_rng(2600)
+    reqs = _requests(rng, 600, max_elems=20000)
+    torch.cuda.synchronize()
+    srv = hptt_amd.Server(slots=3, max_batch=128)
+    out = [None, None]
+    barrier = threading.Barrier(2)
+
+    def client(j):
+        items = (hptt_amd.BatchItem * 300)(*[_item(r) for r in reqs[300 * j:300 * (j + 1)]])
+        barrier.wait()
+        out[j] = srv.submit_many(items)
+
+    ths = [threading.Thread(target=client, args=(j,)) for j in range(2)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    tickets = out[0] + out[1]
+    assert len(set(tickets)) == 600 and 0 not in tickets
+    assert out[0] == sorted(out[0]) and out[1] == sorted(out[1])
+    srv.drain()
+    st = srv.stats()
+    assert st.completed == 600 and st.failed == 0
+    for k, r in enumerate(reqs):
+        c = srv.wait(tickets[k], 5.0)
+        assert c.state == 2, (k, c.status)
+        exp = r["B"].copy()
+        oracle.transpose(r["A"], exp, r["size"], r["perm"], r["alpha"], r["beta"], nthreads=8,
+                         complex_coef=r["cc"])
+        assert oracle.first_mismatch(host(r["dB"]), exp) == -1, k
+    srv.close()
+
+
 def test_server_allocated_outputs(device):
     import torch
     rng = np.random.default_rng(77)
