@@ -338,8 +338,9 @@ def time_serving_manifest(dtype_name, passes=3, seq_reps=3, max_batch=4096, slot
 def time_sweep(reps=10, tuned=True):
     """The reference benchmark's 102 workloads (fp32, alpha=2.3, beta=4.2,
     test/benchmark/inc/hptt/benchmark/benchmark_trans.h:24-140): effective
-    GB/s of plain plan->exec() per shape (median of reps event-timed execs),
-    with the minimum and median over shapes and the slowest shapes, for model
+    GB/s of plain plan->exec() per shape (median of reps event-timed execs
+    run back to back, like the other single-plan workloads), with the minimum
+    and median over shapes and the slowest shapes, for model
     plans (tuning_timeout=0) and, with tuned, for plans autotuned exhaustively
     (tuning_timeout=-1, which is how the reference's benchmark creates its
     plans, test/benchmark/inc/hptt/perf-test/test_perf_util.tcc:22-24)."""
@@ -408,7 +409,13 @@ def time_sweep(reps=10, tuned=True):
            "slowest": [dict(name=k, **v) for k, v in worst], "per_shape": rows}
     if tuned:
         res["tuned"] = {"tuning_timeout": -1.0, "tuning_s_total": round(t_tune, 1),
-                        **summary([r["gbps_tuned"] for r in rows.values()])}
+                        **summary([r["gbps_tuned"] for r in rows.values()]),
+                        "note": "timed back to back like the model's plans; A and B of most "
+                                "of these shapes are about 200 MB each, and the tuner (which "
+                                "times launches back to back too) often picks access hints "
+                                "that leave one of them in the 256 MB last-level cache "
+                                "between runs, a gain that does not carry over to larger "
+                                "problems or a cold cache"}
     return res
 
 

@@ -25,10 +25,11 @@ namespace hpttx {
 
 // One block of NTH threads per tile on a 2-D grid (t = y * gridDim.x + x).
 // UR / UW: unaligned runs on the read / write side (full-width vectors at any
-// element offset, partial last vectors element by element).  NT / NTS: the
-// non-temporal hint on loads (A's and B's) / on stores.
+// element offset, partial last vectors element by element).  NT / NTB / NTS:
+// the non-temporal hint on A's loads / on B's loads (when updating) / on
+// stores.
 template <typename T, typename S, int NC, int VR, int VW, bool UR, bool UW, bool UPD, bool CC,
-          bool SH, bool NT, bool NTS, uint32_t NTH>
+          bool SH, bool NT, bool NTB, bool NTS, uint32_t NTH>
 __global__ __launch_bounds__(NTH) void k_tile(TileDesc d, const T *__restrict__ A,
                                               T *__restrict__ B, Coef<S> co) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -44,9 +45,9 @@ __global__ __launch_bounds__(NTH) void k_tile(TileDesc d, const T *__restrict__ 
   tile_read<T, VR, NT, kReadGroupPlan, UR, NTH>(d, tc, tid, (const HPTTX_GLOBAL T *)A, lds, tb);
   __syncthreads();
   if constexpr (quad_write<T, UW, SH>())
-    tile_write_quad<T, S, NC, UPD, CC, NT, NTH, NTS>(d, tc, tid, (HPTTX_GLOBAL T *)B, lds, tb, co);
+    tile_write_quad<T, S, NC, UPD, CC, NTB, NTH, NTS>(d, tc, tid, (HPTTX_GLOBAL T *)B, lds, tb, co);
   else
-    tile_write<T, S, NC, VW, UPD, CC, SH, NT, kUcapPlan, UW, NTH, NTS>(d, tc, tid,
+    tile_write<T, S, NC, VW, UPD, CC, SH, NTB, kUcapPlan, UW, NTH, NTS>(d, tc, tid,
                                                                        (HPTTX_GLOBAL T *)B, lds, tb, co);
 }
 
@@ -68,7 +69,8 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowDesc d, const T *__restrict_
 
 namespace {
 
-template <typename T, typename S, int NC, int VCAP, bool UPD, bool CC, bool NT, bool NTS>
+template <typename T, typename S, int NC, int VCAP, bool UPD, bool CC, bool NT, bool NTB,
+          bool NTS>
 hipError_t launch_typed(const LaunchPlan &lp, const void *A, void *B,
                         const Coef<S> &co, hipStream_t st) {
   const T *a = static_cast<const T *>(A);
@@ -86,10 +88,10 @@ hipError_t launch_typed(const LaunchPlan &lp, const void *A, void *B,
 #define HPTTX_TILE(UR_, UW_, SH_)                                                              \
   do {                                                                                        \
     if (lp.threads == kWideBlock)                                                             \
-      hipLaunchKernelGGL((k_tile<T, S, NC, VCAP, VCAP, UR_, UW_, UPD, CC, SH_, NT, NTS, kWideBlock>), \
+      hipLaunchKernelGGL((k_tile<T, S, NC, VCAP, VCAP, UR_, UW_, UPD, CC, SH_, NT, NTB, NTS, kWideBlock>), \
                          grid, wide, lp.lds_bytes, st, d, a, b, co);                          \
     else                                                                                      \
-      hipLaunchKernelGGL((k_tile<T, S, NC, VCAP, VCAP, UR_, UW_, UPD, CC, SH_, NT, NTS, kBlock>), \
+      hipLaunchKernelGGL((k_tile<T, S, NC, VCAP, VCAP, UR_, UW_, UPD, CC, SH_, NT, NTB, NTS, kBlock>), \
                          grid, block, lp.lds_bytes, st, d, a, b, co);                         \
   } while (0)
     if (sh) {
@@ -119,30 +121,37 @@ hipError_t launch_typed(const LaunchPlan &lp, const void *A, void *B,
   return hipGetLastError();
 }
 
-template <typename T, typename S, int NC, int VCAP, bool CCOK, bool NT, bool NTS>
+// Without an update B is not read: those launches take NTB = NT, so that no
+// kernel instance differs from another only in a hint it never uses.
+template <typename T, typename S, int NC, int VCAP, bool CCOK, bool NT, bool NTB, bool NTS>
 hipError_t launch_cc(const LaunchPlan &lp, const void *A, void *B, const Coef<S> &co,
                      bool upd, bool cc, hipStream_t st) {
   if (cc && CCOK) {
-    if (upd) return launch_typed<T, S, NC, VCAP, true, CCOK, NT, NTS>(lp, A, B, co, st);
-    return launch_typed<T, S, NC, VCAP, false, CCOK, NT, NTS>(lp, A, B, co, st);
+    if (upd) return launch_typed<T, S, NC, VCAP, true, CCOK, NT, NTB, NTS>(lp, A, B, co, st);
+    return launch_typed<T, S, NC, VCAP, false, CCOK, NT, NT, NTS>(lp, A, B, co, st);
   }
-  if (upd) return launch_typed<T, S, NC, VCAP, true, false, NT, NTS>(lp, A, B, co, st);
-  return launch_typed<T, S, NC, VCAP, false, false, NT, NTS>(lp, A, B, co, st);
+  if (upd) return launch_typed<T, S, NC, VCAP, true, false, NT, NTB, NTS>(lp, A, B, co, st);
+  return launch_typed<T, S, NC, VCAP, false, false, NT, NT, NTS>(lp, A, B, co, st);
 }
 
 template <typename T, typename S, int NC, int VCAP, bool CCOK>
 hipError_t launch_upd(const LaunchPlan &lp, const void *A, void *B, const CoefD &c,
                       bool upd, bool cc, hipStream_t st) {
   Coef<S> co{(S)c.ar, (S)c.ai, (S)c.br, (S)c.bi};
-  // Non-temporal loads / stores: both, loads only (tiles whose runs are not
-  // whole lines, 4-byte elements), or neither; rows set both alike.
+  // Non-temporal hints (A's loads, B's loads, stores): all, none, loads only,
+  // stores only, or stores and B's loads; rows set all alike.  B's loads
+  // carry the hint whenever A's do.
   if (lp.nt_load && lp.nt_store)
-    return launch_cc<T, S, NC, VCAP, CCOK, true, true>(lp, A, B, co, upd, cc, st);
-  if (lp.nt_load && lp.kind == kTile)
-    return launch_cc<T, S, NC, VCAP, CCOK, true, false>(lp, A, B, co, upd, cc, st);
-  if (lp.nt_store && lp.kind == kTile)
-    return launch_cc<T, S, NC, VCAP, CCOK, false, true>(lp, A, B, co, upd, cc, st);
-  return launch_cc<T, S, NC, VCAP, CCOK, false, false>(lp, A, B, co, upd, cc, st);
+    return launch_cc<T, S, NC, VCAP, CCOK, true, true, true>(lp, A, B, co, upd, cc, st);
+  if (lp.kind == kTile) {
+    if (lp.nt_load)
+      return launch_cc<T, S, NC, VCAP, CCOK, true, true, false>(lp, A, B, co, upd, cc, st);
+    if (lp.nt_store && lp.nt_load_b)
+      return launch_cc<T, S, NC, VCAP, CCOK, false, true, true>(lp, A, B, co, upd, cc, st);
+    if (lp.nt_store)
+      return launch_cc<T, S, NC, VCAP, CCOK, false, false, true>(lp, A, B, co, upd, cc, st);
+  }
+  return launch_cc<T, S, NC, VCAP, CCOK, false, false, false>(lp, A, B, co, upd, cc, st);
 }
 
 }  // namespace

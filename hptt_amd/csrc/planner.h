@@ -22,8 +22,8 @@ struct TuneParams {
   uint32_t block_elems = 0;  // target elements per block tile
   uint32_t wide_min_bytes = 0;  // stride-1-kept rows at least this long use ROWS
   uint32_t stream = 0;       // non-temporal accesses: 0 = model, 1 = loads and stores,
-                             // 2 = none, 3 = loads only, 4 = stores only (tiles; rows
-                             // take 3 and 4 as 1)
+                             // 2 = none, 3 = loads only, 4 = stores only, 5 = stores and
+                             // B's loads (tiles; rows take 3 to 5 as 1)
   bool operator==(const TuneParams &o) const {
     return run_r == o.run_r && run_c == o.run_c && block_elems == o.block_elems &&
            wide_min_bytes == o.wide_min_bytes && stream == o.stream;
@@ -73,7 +73,8 @@ struct LaunchPlan {
   uint32_t grid = 0;
   size_t lds_bytes = 0;
   uint32_t threads = kBlock;   // threads per block (kBlock or kWideBlock)
-  bool nt_load = false;        // non-temporal loads (A's, and B's when updating)
+  bool nt_load = false;        // non-temporal loads of A
+  bool nt_load_b = false;      // non-temporal loads of B (when updating)
   bool nt_store = false;       // non-temporal stores
   TuneParams tune{};
   std::string path;            // "tile", "tile-shared", "wide-copy", "axpby", "gather"

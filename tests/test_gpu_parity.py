@@ -310,6 +310,47 @@ def test_concurrent_exec_from_host_threads(device):
         assert oracle.first_mismatch(host(outs[k]), exp) == -1, k
 
 
+def test_every_streaming_mode_is_exact(device, tmp_path):
+    """Tuning-cache records force each streaming mode (1 loads and stores, 2
+    none, 3 loads only, 4 stores only, 5 stores and B's loads) on tile plans
+    with beta != 0, in fp32 and c128; every result equals the oracle.  Runs
+    in a fresh process, which reads the cache file."""
+    import subprocess
+    import sys
+    root = os.path.dirname(HERE)
+    cache = tmp_path / "tune.txt"
+    # {n0, 24, 20, 16} perm {3, 2, 1, 0}: fused dims n0:1:7680, 24:n0:320,
+    # 20:24*n0:16, 16:480*n0:1 (fp32 code 0, c128 code 3, update flag 1)
+    lines = []
+    for mode, n0 in zip(range(1, 6), (64, 32, 96, 128, 160)):
+        fd = "%d:1:7680|24:%d:320|20:%d:16|16:%d:1" % (n0, n0, 24 * n0, 480 * n0)
+        for code in (0, 3):
+            lines.append("gfx950|%d|1|4|%s\t0 0 0 0 %d" % (code, fd, mode))
+    cache.write_text("\n".join(lines) + "\n")
+    script = (
+        "import sys; sys.path.insert(0, 'tests')\n"
+        "import numpy as np, torch, hptt_amd, oracle\n"
+        "names = {1: 'on', 2: 'off', 3: 'loads', 4: 'stores', 5: 'stores+b-loads'}\n"
+        "for mode, n0 in zip(range(1, 6), (64, 32, 96, 128, 160)):\n"
+        "    for dt in (np.float32, np.complex128):\n"
+        "        size, perm = [n0, 24, 20, 16], [3, 2, 1, 0]\n"
+        "        n = int(np.prod(size))\n"
+        "        A = oracle.uniform(n, dt, mode); B = oracle.uniform(n, dt, 10 + mode)\n"
+        "        dA = torch.from_numpy(A).cuda(); dB = torch.from_numpy(B).cuda()\n"
+        "        p = hptt_amd.create_plan_checked(dA, dB, size, perm, 1.5, -0.75)\n"
+        "        d = p.describe()\n"
+        "        assert 'path: tile' in d and ('streaming: %s ' % names[mode]) in d, (mode, d)\n"
+        "        p.exec(); p.sync(); torch.cuda.synchronize()\n"
+        "        exp = B.copy()\n"
+        "        assert oracle.transpose(A, exp, size, perm, 1.5, -0.75) == 0\n"
+        "        assert oracle.first_mismatch(dB.cpu().numpy(), exp) == -1, (mode, dt)\n"
+        "print('MODES OK')\n")
+    env = dict(os.environ, HPTTX_TUNE_CACHE=str(cache))
+    r = subprocess.run([sys.executable, "-c", script], cwd=root, env=env, capture_output=True,
+                       text=True, timeout=110)
+    assert r.returncode == 0 and "MODES OK" in r.stdout, r.stdout + r.stderr
+
+
 def test_bad_tuning_cache_record_falls_back(device, tmp_path):
     """A tuning-cache record that does not plan (here tile runs of 4096
     elements, which need far more than 64 KiB of LDS) is replaced by the

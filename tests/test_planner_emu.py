@@ -182,6 +182,7 @@ def test_streaming_and_tile_order_rules():
         (np.float64, shapes.CONFIG2, "on"),
         (np.complex128, shapes.CONFIG3_TILE, "on"),
         (np.complex128, ([64, 128, 128, 64], [3, 2, 1, 0]), "on"),
+        (np.float32, ([96, 75, 75, 96], [3, 2, 1, 0]), "on"),
         (np.float32, ([97, 97, 97, 97], [3, 2, 1, 0]), "loads"),
         (np.float32, ([2320, 384, 59], [1, 0, 2]), "loads"),
         (np.complex128, ([97, 97, 97, 97], [3, 2, 1, 0]), "off"),
@@ -196,8 +197,9 @@ def test_streaming_and_tile_order_rules():
     # kept rows stream loads and stores alike
     rc, err, d = E.plan(np.complex128, *shapes.CONFIG3_WIDE)
     assert rc == 0 and "streaming: on " in d, d
-    # a tuned choice is kept: loads only (3) and none (2)
-    for st, mode in ((3, "loads"), (2, "off"), (1, "on")):
+    # a tuned choice is kept: loads only (3), none (2), all (1), stores only
+    # (4), stores and B loads (5)
+    for st, mode in ((3, "loads"), (2, "off"), (1, "on"), (4, "stores"), (5, "stores+b-loads")):
         rc, err, d = E.plan(np.float32, *shapes.CONFIG2, tune=[0, 0, 0, 0, st])
         assert rc == 0 and ("streaming: %s " % mode) in d, (st, d)
 
