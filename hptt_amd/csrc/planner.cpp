@@ -510,8 +510,10 @@ void finalize_vectors(const Problem &p, LaunchPlan &lp, uint32_t vr, uint32_t vw
   (void)vw;
 }
 
-bool tile_whole_lines(const Problem &p, const TileDesc &t) {
-  auto lines = [&](uint64_t elems) { return (elems * p.esize) % 128 == 0; };
+// Every run length and extent, and every stride that places a run start,
+// a multiple of `unit` bytes.
+static bool tile_runs_on(const Problem &p, const TileDesc &t, uint64_t unit) {
+  auto lines = [&](uint64_t elems) { return (elems * p.esize) % unit == 0; };
   if (!lines((uint64_t)t.TR * t.VS) || !lines((uint64_t)t.VR * t.VS) ||
       !lines((uint64_t)t.TC * t.VS) || !lines((uint64_t)t.VC * t.VS))
     return false;
@@ -524,16 +526,31 @@ bool tile_whole_lines(const Problem &p, const TileDesc &t) {
   return true;
 }
 
-// Non-temporal choice of a tile plan whose runs are not all whole lines.
-static void stream_partial_lines(const Problem &p, LaunchPlan &lp) {
-  lp.nt_load = lp.nt_load_b = p.esize == 4;
-  lp.nt_store = false;
+bool tile_whole_lines(const Problem &p, const TileDesc &t) { return tile_runs_on(p, t, 128); }
+
+// Each buffer at most the last-level cache's size (A and B together are
+// larger, or nothing streams).
+static bool buffers_fit_cache(const Problem &p) {
+  return std::max(p.spanA, p.spanB) * p.esize <= kLastLevelCacheBytes;
+}
+
+// Non-temporal choice of a tile plan whose runs are not all whole lines:
+// buffers that fit the cache stream loads only (4-byte elements) or nothing;
+// larger ones stream everything for 4-byte elements whose runs all start and
+// end on 64-byte half lines, and nothing otherwise.
+static void stream_partial_lines(const Problem &p, LaunchPlan &lp, bool half_lines) {
+  if (buffers_fit_cache(p)) {
+    lp.nt_load = lp.nt_load_b = p.esize == 4;
+    lp.nt_store = false;
+  } else {
+    lp.nt_load = lp.nt_load_b = lp.nt_store = p.esize == 4 && half_lines;
+  }
 }
 
 void stream_for_pointers(const Problem &p, LaunchPlan &lp, const void *A, const void *B) {
-  if (lp.kind == kTile && lp.nt_store && lp.tune.stream == 0 &&
-      (((uintptr_t)A | (uintptr_t)B) % 128) != 0)
-    stream_partial_lines(p, lp);
+  const uintptr_t a = (uintptr_t)A | (uintptr_t)B;
+  if (lp.kind == kTile && lp.nt_store && lp.tune.stream == 0 && a % 128 != 0)
+    stream_partial_lines(p, lp, a % 64 == 0 && tile_runs_on(p, lp.tile, 64));
 }
 
 int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
@@ -563,23 +580,29 @@ int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
     rc = plan_tile(p, false, tp, min_tiles, lp, err);
   }
   if (rc != kOk) return rc;
-  // Streaming mode.  Model: on when the problem cannot stay in the 256 MiB
-  // last-level cache anyway and every access belongs to a run of at least
-  // 128 bytes; exhaustive tuning of the large 6-D and 4-D workloads chose it
-  // for every winning tile shape, and of the reference's shapes with kept
-  // rows of 128-255 bytes the wide copy with it (192-byte rows: 2.9 ->
-  // 5.3 TB/s against a shared-group tile).  Tiles whose runs are not all
-  // whole 128-byte lines stream loads only (4-byte elements) or nothing
-  // (stream_partial_lines; the pointers' part is checked when the plan gets
-  // its data, stream_for_pointers).  Leaving A's loads (or the stores)
-  // unstreamed measured up to 30% faster on shapes whose A (or B) is at most
-  // the cache's size, run back to back, and not at all (1% slower) on the
-  // same shapes at 2 to 4 times the size: the gain is the cache keeping the
-  // buffer between runs.  The same holds for the partial-line rule: at 675
-  // to 808 MB per buffer, fp32 shapes whose runs are not whole lines ran 10%
-  // to 23% faster with everything streamed (half-line starts) or 24% with
-  // nothing ({194,97,97,97}) than with loads only, which wins at 200 MB.
-  // The tuner may choose any of the modes.
+  // Streaming mode.  Model: on when A and B together exceed the 256 MiB
+  // last-level cache and every access belongs to a run of at least 128
+  // bytes.  Rows (the wide copy) stream every access (of the reference's
+  // shapes with kept rows of 128-255 bytes: 192-byte rows 2.9 -> 5.3 TB/s
+  // against a shared-group tile).  Tiles:
+  //  * runs of whole 128-byte lines: every access streamed; where each
+  //    buffer fits in the cache, A's loads are not, so that A stays there
+  //    for the next call: measured on MI355X, calls repeated back to back,
+  //    the reference's 57 such fp32 shapes (about 200 MB per buffer) +7.5%
+  //    to +30% (23% on average), and with a cold cache (a 512 MB read before
+  //    each call) -0.5% to +0.5%; at 2 to 4 times those sizes the same
+  //    choice lost 1%, and the 6-D {16,24,32,16,24,32} perm {5,2,0,4,1,3}
+  //    (604 MB) 1.2% to 1.7%, so larger buffers stream everything;
+  //  * runs that are not all whole lines (stream_partial_lines; the
+  //    pointers' part is checked when the plan gets its data,
+  //    stream_for_pointers): buffers that fit, loads only (4-byte elements;
+  //    back to back their stores keep B in the cache) or nothing; larger
+  //    ones, everything for 4-byte elements on 64-byte half lines and
+  //    nothing otherwise: of 10 fp32 shapes of 660 MB to 2 GB per buffer,
+  //    +4% to +27% over loads only on every one (everything streamed best on
+  //    3 of 4 on half lines, the fourth 0.8% behind nothing; nothing best on
+  //    5 of 6 off them, the sixth 2.5% behind everything).
+  // The tuner may override.
   const bool rows = lp.kind == kRows;
   if (tp.stream == 1 || (rows && tp.stream >= 3)) {
     lp.nt_load = lp.nt_load_b = lp.nt_store = true;
@@ -603,7 +626,12 @@ int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
     else
       long_runs = lp.rows.ra == 1 && lp.rows.rb == 1 && (uint64_t)lp.rows.L * p.esize >= 128;
     lp.nt_load = lp.nt_load_b = lp.nt_store = long_runs && bytes > kLastLevelCacheBytes;
-    if (lp.nt_load && !rows && !tile_whole_lines(p, lp.tile)) stream_partial_lines(p, lp);
+    if (lp.nt_load && !rows) {
+      if (!tile_whole_lines(p, lp.tile))
+        stream_partial_lines(p, lp, tile_runs_on(p, lp.tile, 64));
+      else if (buffers_fit_cache(p))
+        lp.nt_load = false;   // A stays in the cache for the next call (tune stream 5)
+    }
   }
   finalize_vectors(p, lp, lp.vr_shape, lp.vw_shape);
   if (lp.lds_bytes > 65536) { err = "tile needs more than 64 KiB of LDS"; return kEUnsupported; }

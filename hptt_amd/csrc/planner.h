@@ -112,17 +112,14 @@ int plan_launch(const Problem &p, const TuneParams &tp, LaunchPlan &lp,
 // unaligned full-width vectors (scalars for 16-byte elements).
 void finalize_vectors(const Problem &p, LaunchPlan &lp, uint32_t vr, uint32_t vw);
 
-// Streaming (non-temporal) accesses of a tile plan: loads and stores where
-// every run starts and ends on a 128-byte line; where some do not (a run
-// that ends inside a line leaves the rest of the line to a neighbouring
-// tile), loads only for 4-byte elements and none for wider ones.  Measured
-// on MI355X (problems beyond the last-level cache, against loads and stores
-// streamed): the fp32 {3,2,1,0} reversal of {97,97,97,97} 3.41 -> 4.32 TB/s
-// with loads only, the reference's {48,352,4,28,28} perm {1,3,0,4,2} 5.87 ->
-// 6.21 and {2320,384,59} perm {1,0,2} 6.19 -> 6.24; the c128 reversals of
-// {97,97,97,97} 3.68 -> 4.16 and {90,96,96,110} 4.02 -> 4.51 with none (loads
-// only: 3.41 and 3.66); on runs of whole lines both stay 5% ahead of either
-// alone ({96,96,96,96} and {64,128,128,64} c128).
+// Streaming (non-temporal) accesses of a tile plan (plan_launch gives the
+// measurements): where every run starts and ends on a 128-byte line, every
+// access, except A's loads when each buffer fits in the last-level cache (A
+// then stays there for the next call); where some run does not (it leaves
+// the rest of its line to a neighbouring tile), loads only (4-byte
+// elements) or nothing when each buffer fits, and for larger buffers
+// everything (4-byte elements whose runs start and end on 64-byte half
+// lines) or nothing.
 // tile_whole_lines: the shape's part (run lengths and the strides that place
 // run starts); stream_for_pointers: the same choice for pointers that are
 // not 128-byte aligned (a tuned choice is kept).

@@ -173,18 +173,33 @@ def test_plans_for_benchmark_shapes():
 
 
 def test_streaming_and_tile_order_rules():
-    """Problems beyond the last-level cache: XCD-aware tile order always;
-    non-temporal loads and stores where every run covers whole 128-byte
-    lines, loads only (4-byte elements) or none (wider) where some run does
-    not; problems the cache holds stream nothing and keep the plain order."""
+    """Problems beyond the last-level cache (A and B together): XCD-aware
+    tile order always.  Runs of whole 128-byte lines: every access streamed,
+    except A's loads where each buffer fits in the cache.  Runs that are not
+    all whole lines: where each buffer fits, loads only (4-byte elements) or
+    none (wider); larger buffers, everything for 4-byte elements on 64-byte
+    half lines, none otherwise.  Problems the cache holds stream nothing
+    and keep the plain order."""
     cases = [
+        # whole lines, buffers of 604 MB to 1.4 GB
         (np.float32, shapes.CONFIG2, "on"),
         (np.float64, shapes.CONFIG2, "on"),
         (np.complex128, shapes.CONFIG3_TILE, "on"),
         (np.complex128, ([64, 128, 128, 64], [3, 2, 1, 0]), "on"),
-        (np.float32, ([96, 75, 75, 96], [3, 2, 1, 0]), "on"),
-        (np.float32, ([97, 97, 97, 97], [3, 2, 1, 0]), "loads"),
+        (np.float32, ([192, 75, 75, 96], [3, 2, 1, 0]), "on"),
+        # whole lines, buffers of 198 and 200 MB
+        (np.float32, ([96, 75, 75, 96], [3, 2, 1, 0]), "stores+b-loads"),
+        (np.float32, ([384, 355, 384], [2, 1, 0]), "stores+b-loads"),
+        # partial lines, buffers of 200 MB
         (np.float32, ([2320, 384, 59], [1, 0, 2]), "loads"),
+        (np.float32, ([7248, 7248], [1, 0]), "loads"),
+        # partial lines, larger buffers: half lines (rows of 9280 bytes) ...
+        (np.float32, ([2320, 384, 236], [1, 0, 2]), "on"),
+        (np.float32, ([7248, 24000], [1, 0]), "on"),
+        # ... and less (388, 400 and 19248 bytes)
+        (np.float32, ([97, 97, 97, 97], [3, 2, 1, 0]), "off"),
+        (np.float32, ([100, 100, 100, 200], [3, 2, 1, 0]), "off"),
+        (np.float32, ([4812, 36000], [1, 0]), "off"),
         (np.complex128, ([97, 97, 97, 97], [3, 2, 1, 0]), "off"),
         (np.float64, ([100, 100, 100, 100], [3, 2, 1, 0]), "off"),
     ]
