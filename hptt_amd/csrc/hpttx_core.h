@@ -774,9 +774,10 @@ HPTTX_HD void row_tables(const RowDesc &d, const RowCoord &rc, uint32_t tid,
 // in a partial vector, which is stored element by element.  Loads go through
 // load_item (no branch per item: items past the block load the segment's
 // first vector; a partial last vector is loaded as the full vector ending at
-// the segment's end).  UA rows are contiguous (ra = rb = 1).
+// the segment's end).  UA rows are contiguous (ra = rb = 1).  NTA: the
+// non-temporal hint on A's loads; NT: on B's loads and the stores.
 template <typename T, typename S, int NC, int V, bool UPD, bool CC, bool NT, int UCAP, bool UA,
-          bool WHOLE>
+          bool WHOLE, bool NTA>
 HPTTX_HD void row_pass(const RowDesc &d, const RowCoord &rc, uint32_t tid,
                        const HPTTX_GLOBAL T *HPTTX_RESTRICT A, HPTTX_GLOBAL T *HPTTX_RESTRICT B,
                        const int64_t *tabA, const int64_t *tabB, const Coef<S> &k) {
@@ -801,7 +802,7 @@ HPTTX_HD void row_pass(const RowDesc &d, const RowCoord &rc, uint32_t tid,
       const int64_t e = (int64_t)rc.seg0 + (int64_t)(ok ? v : 0) * V;
       const uint32_t rs = ok ? rr : 0;
       ob[u] = tabB[rs] + e * d.rb;
-      load_item<T, V, NT, UA, WHOLE>(A, tabA[rs] + e * d.ra, n[u], afirst, x[u]);
+      load_item<T, V, NTA, UA, WHOLE>(A, tabA[rs] + e * d.ra, n[u], afirst, x[u]);
       if (UPD) load_item<T, V, NT, UA, WHOLE>(B, ob[u], n[u], bfirst, y[u]);
     }
 #pragma unroll
@@ -833,15 +834,15 @@ HPTTX_HD void row_pass(const RowDesc &d, const RowCoord &rc, uint32_t tid,
 }
 
 template <typename T, typename S, int NC, int V, bool UPD, bool CC, bool NT, int UCAP,
-          bool UA = false>
+          bool UA = false, bool NTA = NT>
 HPTTX_HD void row_body(const RowDesc &d, const RowCoord &rc, uint32_t tid,
                        const HPTTX_GLOBAL T *HPTTX_RESTRICT A,
                        HPTTX_GLOBAL T *HPTTX_RESTRICT B,
                        const int64_t *tabA, const int64_t *tabB, const Coef<S> &k) {
   if (!UA || rc.slen >= (uint32_t)V)
-    row_pass<T, S, NC, V, UPD, CC, NT, UCAP, UA, true>(d, rc, tid, A, B, tabA, tabB, k);
+    row_pass<T, S, NC, V, UPD, CC, NT, UCAP, UA, true, NTA>(d, rc, tid, A, B, tabA, tabB, k);
   else
-    row_pass<T, S, NC, V, UPD, CC, NT, UCAP, UA, false>(d, rc, tid, A, B, tabA, tabB, k);
+    row_pass<T, S, NC, V, UPD, CC, NT, UCAP, UA, false, NTA>(d, rc, tid, A, B, tabA, tabB, k);
 }
 
 }  // namespace hpttx

@@ -51,7 +51,8 @@ __global__ __launch_bounds__(NTH) void k_tile(TileDesc d, const T *__restrict__ 
                                                                        (HPTTX_GLOBAL T *)B, lds, tb, co);
 }
 
-template <typename T, typename S, int NC, int V, bool UA, bool UPD, bool CC, bool NT>
+// NTA / NT: the non-temporal hint on A's loads / on B's loads and the stores.
+template <typename T, typename S, int NC, int V, bool UA, bool UPD, bool CC, bool NTA, bool NT>
 __global__ __launch_bounds__(kBlock) void k_rows(RowDesc d, const T *__restrict__ A,
                                                  T *__restrict__ B, Coef<S> co) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -63,8 +64,8 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowDesc d, const T *__restrict_
   const RowCoord rc = row_coord(d, t);
   row_tables(d, rc, tid, tabA, tabB);
   __syncthreads();
-  row_body<T, S, NC, V, UPD, CC, NT, kUcapPlan, UA>(d, rc, tid, (const HPTTX_GLOBAL T *)A,
-                                                   (HPTTX_GLOBAL T *)B, tabA, tabB, co);
+  row_body<T, S, NC, V, UPD, CC, NT, kUcapPlan, UA, NTA>(d, rc, tid, (const HPTTX_GLOBAL T *)A,
+                                                        (HPTTX_GLOBAL T *)B, tabA, tabB, co);
 }
 
 namespace {
@@ -108,15 +109,17 @@ hipError_t launch_typed(const LaunchPlan &lp, const void *A, void *B,
 #undef HPTTX_TILE
   } else {
     const RowDesc &d = lp.rows;
+    // rows: A's loads take NT, B's loads and the stores NTS (the planner
+    // gives rows all, none, or stores and B's loads)
     if (d.v == VCAP && VCAP > 1 && !d.ua)
-      hipLaunchKernelGGL((k_rows<T, S, NC, VCAP, false, UPD, CC, NT>), grid, block,
+      hipLaunchKernelGGL((k_rows<T, S, NC, VCAP, false, UPD, CC, NT, NTS>), grid, block,
                          lp.lds_bytes, st, d, a, b, co);
     else if (d.v == VCAP && VCAP > 1)
-      hipLaunchKernelGGL((k_rows<T, S, NC, VCAP, kUA, UPD, CC, NT>), grid, block,
+      hipLaunchKernelGGL((k_rows<T, S, NC, VCAP, kUA, UPD, CC, NT, NTS>), grid, block,
                          lp.lds_bytes, st, d, a, b, co);
     else
-      hipLaunchKernelGGL((k_rows<T, S, NC, 1, false, UPD, CC, NT>), grid, block, lp.lds_bytes,
-                         st, d, a, b, co);
+      hipLaunchKernelGGL((k_rows<T, S, NC, 1, false, UPD, CC, NT, NTS>), grid, block,
+                         lp.lds_bytes, st, d, a, b, co);
   }
   return hipGetLastError();
 }
@@ -138,16 +141,16 @@ template <typename T, typename S, int NC, int VCAP, bool CCOK>
 hipError_t launch_upd(const LaunchPlan &lp, const void *A, void *B, const CoefD &c,
                       bool upd, bool cc, hipStream_t st) {
   Coef<S> co{(S)c.ar, (S)c.ai, (S)c.br, (S)c.bi};
-  // Non-temporal hints (A's loads, B's loads, stores): all, none, loads only,
-  // stores only, or stores and B's loads; rows set all alike.  B's loads
-  // carry the hint whenever A's do.
+  // Non-temporal hints (A's loads, B's loads, stores): all, none, or stores
+  // and B's loads; tiles also loads only and stores only.  B's loads carry
+  // the hint whenever A's do.
   if (lp.nt_load && lp.nt_store)
     return launch_cc<T, S, NC, VCAP, CCOK, true, true, true>(lp, A, B, co, upd, cc, st);
+  if (lp.nt_store && lp.nt_load_b)
+    return launch_cc<T, S, NC, VCAP, CCOK, false, true, true>(lp, A, B, co, upd, cc, st);
   if (lp.kind == kTile) {
     if (lp.nt_load)
       return launch_cc<T, S, NC, VCAP, CCOK, true, true, false>(lp, A, B, co, upd, cc, st);
-    if (lp.nt_store && lp.nt_load_b)
-      return launch_cc<T, S, NC, VCAP, CCOK, false, true, true>(lp, A, B, co, upd, cc, st);
     if (lp.nt_store)
       return launch_cc<T, S, NC, VCAP, CCOK, false, false, true>(lp, A, B, co, upd, cc, st);
   }

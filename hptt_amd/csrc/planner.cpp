@@ -584,7 +584,9 @@ int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
   // last-level cache and every access belongs to a run of at least 128
   // bytes.  Rows (the wide copy) stream every access (of the reference's
   // shapes with kept rows of 128-255 bytes: 192-byte rows 2.9 -> 5.3 TB/s
-  // against a shared-group tile).  Tiles:
+  // against a shared-group tile), except A's loads where each buffer fits
+  // in the cache (the reference's 12 such row shapes +11% to +20% back to
+  // back, -1.6% to +5% with a cold cache).  Tiles:
   //  * runs of whole 128-byte lines: every access streamed; where each
   //    buffer fits in the cache, A's loads are not, so that A stays there
   //    for the next call: measured on MI355X, calls repeated back to back,
@@ -604,7 +606,7 @@ int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
   //    5 of 6 off them, the sixth 2.5% behind everything).
   // The tuner may override.
   const bool rows = lp.kind == kRows;
-  if (tp.stream == 1 || (rows && tp.stream >= 3)) {
+  if (tp.stream == 1 || (rows && (tp.stream == 3 || tp.stream == 4))) {
     lp.nt_load = lp.nt_load_b = lp.nt_store = true;
   } else if (tp.stream == 2) {
     lp.nt_load = lp.nt_load_b = lp.nt_store = false;
@@ -626,8 +628,8 @@ int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
     else
       long_runs = lp.rows.ra == 1 && lp.rows.rb == 1 && (uint64_t)lp.rows.L * p.esize >= 128;
     lp.nt_load = lp.nt_load_b = lp.nt_store = long_runs && bytes > kLastLevelCacheBytes;
-    if (lp.nt_load && !rows) {
-      if (!tile_whole_lines(p, lp.tile))
+    if (lp.nt_load) {
+      if (!rows && !tile_whole_lines(p, lp.tile))
         stream_partial_lines(p, lp, tile_runs_on(p, lp.tile, 64));
       else if (buffers_fit_cache(p))
         lp.nt_load = false;   // A stays in the cache for the next call (tune stream 5)

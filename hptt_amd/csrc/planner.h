@@ -23,7 +23,7 @@ struct TuneParams {
   uint32_t wide_min_bytes = 0;  // stride-1-kept rows at least this long use ROWS
   uint32_t stream = 0;       // non-temporal accesses: 0 = model, 1 = loads and stores,
                              // 2 = none, 3 = loads only, 4 = stores only, 5 = stores and
-                             // B's loads (tiles; rows take 3 to 5 as 1)
+                             // B's loads (rows take 3 and 4 as 1)
   bool operator==(const TuneParams &o) const {
     return run_r == o.run_r && run_c == o.run_c && block_elems == o.block_elems &&
            wide_min_bytes == o.wide_min_bytes && stream == o.stream;

@@ -312,9 +312,10 @@ def test_concurrent_exec_from_host_threads(device):
 
 def test_every_streaming_mode_is_exact(device, tmp_path):
     """Tuning-cache records force each streaming mode (1 loads and stores, 2
-    none, 3 loads only, 4 stores only, 5 stores and B's loads) on tile plans
-    with beta != 0, in fp32 and c128; every result equals the oracle.  Runs
-    in a fresh process, which reads the cache file."""
+    none, 3 loads only, 4 stores only, 5 stores and B's loads) on tile plans,
+    and mode 5 on a row plan, with beta != 0, in fp32 and c128; every result
+    equals the oracle.  Runs in a fresh process, which reads the cache
+    file."""
     import subprocess
     import sys
     root = os.path.dirname(HERE)
@@ -326,6 +327,9 @@ def test_every_streaming_mode_is_exact(device, tmp_path):
         fd = "%d:1:7680|24:%d:320|20:%d:16|16:%d:1" % (n0, n0, 24 * n0, 480 * n0)
         for code in (0, 3):
             lines.append("gfx950|%d|1|4|%s\t0 0 0 0 %d" % (code, fd, mode))
+    # {64, 24, 20, 16} perm {0, 3, 2, 1}: rows of 64 along the kept dim
+    for code in (0, 3):
+        lines.append("gfx950|%d|1|4|64:1:1|24:64:20480|20:1536:1024|16:30720:64\t0 0 0 0 5" % code)
     cache.write_text("\n".join(lines) + "\n")
     script = (
         "import sys; sys.path.insert(0, 'tests')\n"
@@ -344,6 +348,19 @@ def test_every_streaming_mode_is_exact(device, tmp_path):
         "        exp = B.copy()\n"
         "        assert oracle.transpose(A, exp, size, perm, 1.5, -0.75) == 0\n"
         "        assert oracle.first_mismatch(dB.cpu().numpy(), exp) == -1, (mode, dt)\n"
+        "# rows (the stride-1 dim kept) under stores and B's loads\n"
+        "size, perm = [64, 24, 20, 16], [0, 3, 2, 1]\n"
+        "n = int(np.prod(size))\n"
+        "for dt in (np.float32, np.complex128):\n"
+        "    A = oracle.uniform(n, dt, 21); B = oracle.uniform(n, dt, 22)\n"
+        "    dA = torch.from_numpy(A).cuda(); dB = torch.from_numpy(B).cuda()\n"
+        "    p = hptt_amd.create_plan_checked(dA, dB, size, perm, 1.5, -0.75)\n"
+        "    d = p.describe()\n"
+        "    assert 'path: wide-copy' in d and 'streaming: stores+b-loads ' in d, d\n"
+        "    p.exec(); p.sync(); torch.cuda.synchronize()\n"
+        "    exp = B.copy()\n"
+        "    assert oracle.transpose(A, exp, size, perm, 1.5, -0.75) == 0\n"
+        "    assert oracle.first_mismatch(dB.cpu().numpy(), exp) == -1, ('rows', dt)\n"
         "print('MODES OK')\n")
     env = dict(os.environ, HPTTX_TUNE_CACHE=str(cache))
     r = subprocess.run([sys.executable, "-c", script], cwd=root, env=env, capture_output=True,

@@ -209,9 +209,12 @@ def test_streaming_and_tile_order_rules():
         assert ("streaming: %s " % mode) in d and "xcd-order" in d, (size, perm, d)
     rc, err, d = E.plan(np.float32, *shapes.CONFIG1)
     assert rc == 0 and "streaming: off " in d and "xcd-order" not in d, d
-    # kept rows stream loads and stores alike
+    # kept rows stream every access (1.4 GB per buffer), except A's loads
+    # where each buffer fits in the cache (200 MB)
     rc, err, d = E.plan(np.complex128, *shapes.CONFIG3_WIDE)
     assert rc == 0 and "streaming: on " in d, d
+    rc, err, d = E.plan(np.float32, [368, 384, 384], [0, 2, 1])
+    assert rc == 0 and "path: wide-copy" in d and "streaming: stores+b-loads " in d, d
     # a tuned choice is kept: loads only (3), none (2), all (1), stores only
     # (4), stores and B loads (5)
     for st, mode in ((3, "loads"), (2, "off"), (1, "on"), (4, "stores"), (5, "stores+b-loads")):
