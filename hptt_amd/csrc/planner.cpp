@@ -591,8 +591,9 @@ int plan_launch(const Problem &p, const TuneParams &tp_in, LaunchPlan &lp,
   //    buffer fits in the cache, A's loads are not, so that A stays there
   //    for the next call: measured on MI355X, calls repeated back to back,
   //    the reference's 57 such fp32 shapes (about 200 MB per buffer) +7.5%
-  //    to +30% (23% on average), and with a cold cache (a 512 MB read before
-  //    each call) -0.5% to +0.5%; at 2 to 4 times those sizes the same
+  //    to +30% (23% on average), fp64, c64 and c128 reversals of 198 MB
+  //    +17% to +22%, and with a cold cache (a 512 MB read before each call)
+  //    -0.9% to +0.5%; at 2 to 4 times those sizes the same
   //    choice lost 1%, and the 6-D {16,24,32,16,24,32} perm {5,2,0,4,1,3}
   //    (604 MB) 1.2% to 1.7%, so larger buffers stream everything;
   //  * runs that are not all whole lines (stream_partial_lines; the
