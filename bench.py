@@ -103,7 +103,8 @@ def time_workload(name, steps, warmup, tune=0.0):
     info = plan.info()
     log(plan.describe().rstrip())
     nbytes = info.algorithmic_bytes
-    flush = torch.empty(512 * 2**20 // 4, device="cuda", dtype=torch.float32) if w["flush"] else None
+    flush = (torch.ones(512 * 2**20 // 4, device="cuda", dtype=torch.float32) if w["flush"]
+             else None)
     for _ in range(warmup):
         plan.exec()
     plan.sync()
@@ -120,13 +121,16 @@ def time_workload(name, steps, warmup, tune=0.0):
         wall = time.perf_counter() - t0
         ms = e0.elapsed_time(e1) / steps
     else:
-        # Data (16.8 MB) would sit in the 256 MB last-level cache: overwrite a
+        # Data (16.8 MB) would sit in the 256 MB last-level cache: read a
         # 512 MB buffer before each launch and time each launch on its own.
+        # A read leaves the cache cold and clean; writing the buffer instead
+        # would leave it full of dirty lines, whose write-back the timed
+        # launch would then pay for (1.38 against 1.46 TB/s).
         evs = []
         t0 = time.perf_counter()
         for _ in range(steps):
             with torch.cuda.stream(stream):
-                flush.fill_(1.0)
+                flush.sum()
             a = torch.cuda.Event(enable_timing=True)
             b = torch.cuda.Event(enable_timing=True)
             a.record(stream)
