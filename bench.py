@@ -687,8 +687,8 @@ def main():
                          "for that layout), serving or sweep" % ", ".join(WORKLOADS))
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how a transposition would shard)", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(a transposition is measured on one GPU's share)", file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():

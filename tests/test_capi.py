@@ -129,8 +129,8 @@ def test_cpp_dropin_header_compiles(tmp_path):
     assert r.returncode == 0, r.stderr
 
 
-# Where the reference's public headers are in this build container (absent on
-# machines that only run the library: the test then skips).
+# Where the reference's public headers are, on a machine that has its sources
+# (elsewhere the test skips).
 REFERENCE_INC = "/root/reference/inc"
 
 # The mangled factory names the reference's front end looks up in its plugin
