@@ -410,6 +410,11 @@ def time_sweep(reps=10, tuned=True):
     del Abuf, Bbuf
     torch.cuda.empty_cache()
     res = {"shapes": len(rows), **summary([r["gbps"] for r in rows.values()]),
+           "note": "execs timed back to back; most of these shapes have A and B of about 200 MB "
+                   "each, and where each buffer fits in the 256 MB last-level cache the planner "
+                   "leaves A's loads unstreamed (no cost with a cold cache), so A stays in the "
+                   "cache between runs: those rates count algorithmic bytes, part of which the "
+                   "cache serves",
            "slowest": [dict(name=k, **v) for k, v in worst], "per_shape": rows}
     if tuned:
         res["tuned"] = {"tuning_timeout": -1.0, "tuning_s_total": round(t_tune, 1),
